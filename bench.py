@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Throughput benchmark: converged power-flow solves per second at 1e-8 pu.
+
+Workload (one GPU): a batch of 4,096 independent grids, sizes log-uniform in
+[50, 5000] buses (random tree + 40 extra edges, loads x U[0.7, 1.3] x 0.5), solved
+together in one plan with FP32 factorisation and FP64 residuals.  A step is one
+full batched solve from flat start (Y-bus assembly + damped Newton-Raphson to
+max|F| < 1e-8 pu on every grid); inputs are resident in HBM before timing.
+
+Also reported:
+  * plain_fp64: the same grids solved one at a time with the FP64 path (a
+    stratified sample of the batch, plans built before timing) -> solves/s and
+    the batched/plain ratio;
+  * roofline of the dominant kernel family: algorithmic bytes per launch over
+    its mean launch time (HIP events around every launch, on the plan's stream);
+  * cpu_baseline: the FP64 C oracle (pforacle.c, 1 thread) on a stratified
+    sample of the same grids.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grids 4096]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "converged PF solves/sec @1e-8 pu: mixed/batched vs plain FP64 GPU NR"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stratified(sizes, count):
+    order = np.argsort(sizes, kind="stable")
+    pick = np.linspace(0, len(order) - 1, count).round().astype(int)
+    return order[pick]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grids", type=int, default=4096)
+    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=1536)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batch shards by "
+              "independent grids with no exchange, so N GPUs = N independent runs.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from gap_amd import capi, gridgen
+
+    if not torch.cuda.is_available() or not capi.device_available():
+        raise SystemExit("bench.py needs a gfx950 GPU and the built HIP library")
+    torch.cuda.set_device(0)
+
+    tol, max_it, base = 1e-8, 100, 100e6
+    t0 = time.time()
+    nets = gridgen.batch_grids(args.grids)
+    sizes = np.array([n.n_bus for n in nets])
+    log(f"[bench] generated {len(nets)} grids, {sizes.sum()} buses in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    plan = capi.Plan(nets, "mixed")
+    st = plan.stats()
+    log(f"[bench] plan: {time.time() - t0:.1f}s (analysis {st['analysis_ms']:.0f} ms), fronts {st['n_fronts']}, "
+        f"levels {st['n_levels']}, max front {st['max_front']}")
+
+    for _ in range(args.warmup):
+        r = plan.solve(tol, max_it, base, fetch_v=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r = plan.solve(tol, max_it, base, fetch_v=False)
+        log(f"[bench] step {k + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    its = r["iterations"]
+    if not r["converged"].all():
+        raise SystemExit(f"{(~r['converged']).sum()} grids did not converge; the number would be invalid")
+    value = len(nets) * args.steps / dt
+    ms_per_step = 1e3 * dt / args.steps
+    log(f"[bench] batched mixed: {value:.0f} solves/s ({ms_per_step:.2f} ms/step), iterations p50 "
+        f"{np.percentile(its, 50):.0f} p99 {np.percentile(its, 99):.0f}")
+
+    # accuracy of the mixed path against the FP64 path on the same batch
+    r_mixed = plan.solve(tol, max_it, base, fetch_v=True)
+    plan64 = capi.Plan(nets, "fp64")
+    r64 = plan64.solve(tol, max_it, base, fetch_v=True)
+    dv = max(float(np.abs(a - b).max()) for a, b in zip(r_mixed["v"], r64["v"]))
+    it_diff = int(np.abs(r_mixed["iterations"].astype(int) - r64["iterations"].astype(int)).max())
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        plan64.solve(tol, max_it, base, fetch_v=False)
+    torch.cuda.synchronize()
+    batched_fp64 = len(nets) * max(1, args.steps // 2) / (time.perf_counter() - t0)
+    del plan64, r64, r_mixed
+
+    # plain FP64 single-instance GPU path on a stratified sample of the same grids
+    idx = stratified(sizes, min(args.plain_sample, len(nets)))
+    plans = [capi.Plan([nets[i]], "fp64") for i in idx]
+    for p in plans[:4]:
+        p.solve(tol, max_it, base, fetch_v=False)
+    times = []
+    for p in plans:
+        t1 = time.perf_counter()
+        p.solve(tol, max_it, base, fetch_v=False)
+        times.append(time.perf_counter() - t1)
+    plain = len(times) / sum(times)
+    del plans
+    log(f"[bench] plain FP64 single-instance: {plain:.1f} solves/s (sample of {len(times)}); "
+        f"batched FP64 {batched_fp64:.0f} solves/s")
+
+    # live per-kernel timing (HIP events around every launch on the plan's stream)
+    plan.profile(True)
+    plan.solve(tol, max_it, base, fetch_v=False)
+    rep = plan.profile_report()
+    plan.profile(False)
+    fam = max(rep, key=lambda k: rep[k]["ms"])  # dominant kernel by device time per step
+    f = rep[fam]
+    avg_ms = f["ms"] / max(f["launches"], 1)
+    bytes_per_launch = f["bytes"] / max(f["launches"], 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": fam, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "avg_launch_us": round(avg_ms * 1e3, 3), "launches_per_step": f["launches"],
+            "algorithmic_bytes_per_launch": round(bytes_per_launch),
+            "kernels_ms_per_step": {k: round(v["ms"], 3) for k, v in rep.items() if v["launches"]}}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(nets, sizes, args.cpu_sample, tol, max_it, base)
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 factor+solve / fp64 residual, update, convergence", "data": "synthetic",
+        "config": {"workload": f"batch of {len(nets)} grids, 50-5000 buses log-uniform (tree + 40 extra "
+                               "edges), loads x U[0.7,1.3] x 0.5, one plan, mixed precision",
+                   "grids": len(nets), "buses": int(sizes.sum()), "tolerance_pu": tol,
+                   "base_power_va": base, "max_iterations": max_it},
+        "plain_fp64_solves_per_s": round(plain, 2),
+        "speedup_vs_plain_fp64": round(value / plain, 2),
+        "batched_fp64_solves_per_s": round(batched_fp64, 1),
+        "iterations_p50": float(np.percentile(its, 50)), "iterations_p99": float(np.percentile(its, 99)),
+        "mixed_vs_fp64_max_abs_dv": dv, "mixed_vs_fp64_max_iteration_diff": it_diff,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(nets, sizes, count, tol, max_it, base):
+    """FP64 C oracle, one thread, on a stratified sample of the batch."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    if not pyoracle.ORACLE_SO.exists():
+        return None
+    o = pyoracle.Oracle()
+    idx = stratified(sizes, min(count, len(nets)))
+    t0 = time.perf_counter()
+    for i in idx:
+        o.solve(nets[i], tol=tol, max_it=max_it, base_power=base, j_call=-1, traj=False)
+    dt = time.perf_counter() - t0
+    return {"value": round(len(idx) / dt, 3), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"{len(idx)} grids stratified by size from the same batch "
+                      f"({int(sizes[idx].sum())} buses), FP64 C oracle (pforacle.c), {dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

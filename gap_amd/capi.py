@@ -1,0 +1,331 @@
+"""ctypes binding of the C ABI in include/gap_hip.h (libgap_hip.so).
+
+``Plan`` wraps a batched power-flow plan; ``LU`` the general sparse LU.
+The shared library is loaded from gap_amd/lib/; a missing or stale build is an
+error, never a silent fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Sequence
+
+import numpy as np
+
+from .network import Network
+
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+CORE_SO = LIB_DIR / "libgap_hip.so"
+PLUGINS_SO = LIB_DIR / "libgap_hip_plugins.so"
+
+OK, E_ARG, E_DEVICE, E_CAPACITY, E_SINGULAR, E_STATE = 0, -1, -2, -3, -4, -5
+FP64, MIXED = 0, 1
+
+_i = C.POINTER(C.c_int)
+_d = C.POINTER(C.c_double)
+_ll = C.POINTER(C.c_longlong)
+
+
+class GapHipError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where}: {msg} (code {code})")
+        self.code = code
+
+
+class GaphipNet(C.Structure):
+    _fields_ = [
+        ("n_bus", C.c_int), ("bus_id", _i), ("u_rated", _d), ("bus_type", _i), ("u_pu", _d),
+        ("p_w", _d), ("q_var", _d),
+        ("n_branch", C.c_int), ("br_id", _i), ("br_from", _i), ("br_to", _i), ("br_status", _i),
+        ("br_r", _d), ("br_x", _d), ("br_g", _d), ("br_b", _d),
+        ("n_app", C.c_int), ("app_id", _i), ("app_node", _i), ("app_status", _i), ("app_type", _i),
+        ("app_p", _d), ("app_q", _d), ("app_g", _d), ("app_b", _d),
+    ]
+
+
+class PlanStats(C.Structure):
+    _fields_ = [("n_inst", C.c_int), ("n_bus", C.c_longlong), ("n_branch", C.c_longlong),
+                ("ybus_nnz", C.c_longlong), ("n_var", C.c_longlong), ("jac_nnz", C.c_longlong),
+                ("n_fronts", C.c_longlong), ("n_levels", C.c_longlong),
+                ("front_entries", C.c_longlong), ("factor_entries", C.c_longlong),
+                ("max_front", C.c_longlong), ("analysis_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int), ("base_power", C.c_double)]
+
+
+# every function the header declares, with its ctypes signature
+SIGNATURES = {
+    "gaphip_abi_version": (C.c_int, []),
+    "gaphip_last_error": (C.c_char_p, []),
+    "gaphip_device_available": (C.c_int, []),
+    "gaphip_ybus_build": (C.c_int, [C.POINTER(GaphipNet), _i, _i, _d, C.c_int, _i]),
+    "gaphip_plan_create": (C.c_int, [C.c_int, C.POINTER(GaphipNet), C.c_int, C.POINTER(C.c_void_p)]),
+    "gaphip_plan_destroy": (C.c_int, [C.c_void_p]),
+    "gaphip_analyze": (C.c_int, [C.c_int, C.POINTER(GaphipNet), C.POINTER(PlanStats)]),
+    "gaphip_plan_stats_get": (C.c_int, [C.c_void_p, C.POINTER(PlanStats)]),
+    "gaphip_plan_set_injections": (C.c_int, [C.c_void_p, C.c_int, _d, _d]),
+    "gaphip_plan_set_branch_rx": (C.c_int, [C.c_void_p, C.c_int, _d, _d]),
+    "gaphip_plan_set_ybus": (C.c_int, [C.c_void_p, C.c_int, _i, _i, _d, C.c_int, C.c_int]),
+    "gaphip_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), _d, _i, _i, _d]),
+    "gaphip_trajectory": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_int, _i]),
+    "gaphip_jacobian": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_double, _i, _i, _i, _d, C.c_int, _i]),
+    "gaphip_mismatches": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_double, _d, C.c_int, _i]),
+    "gaphip_adjoint": (C.c_int, [C.c_void_p, C.c_double, C.c_double, _d, _d, _d, _d, _d, _d, _d]),
+    "gaphip_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "gaphip_profile_get": (C.c_int, [C.c_void_p, C.c_int, _d, _ll, _d]),
+    "gaphip_kernel_name": (C.c_char_p, [C.c_int]),
+    "gaphip_kernel_count": (C.c_int, []),
+    "gaphip_lu_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "gaphip_lu_destroy": (C.c_int, [C.c_void_p]),
+    "gaphip_lu_factorize": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i, _i, _d, C.c_int]),
+    "gaphip_lu_solve": (C.c_int, [C.c_void_p, _d, _d, C.c_int]),
+    "gaphip_lu_is_factorized": (C.c_int, [C.c_void_p]),
+}
+PLUGIN_SYMBOLS = ("create_gpu_admittance_matrix", "create_gpu_lu_solver", "create_gpu_powerflow_solver")
+
+_lib = None
+
+
+def lib():
+    """The loaded C ABI library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not CORE_SO.exists():
+            raise ImportError(f"{CORE_SO} is missing: build with `python -m gap_amd.build`")
+        L = C.CDLL(str(CORE_SO))
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, where: str):
+    if rc != OK:
+        raise GapHipError(rc, where, lib().gaphip_last_error().decode())
+
+
+def device_available() -> bool:
+    return bool(lib().gaphip_device_available())
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def make_net(net: Network):
+    keep = []
+
+    def I(a):
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        keep.append(a)
+        return _p(a, _i)
+
+    def D(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _p(a, _d)
+
+    s = GaphipNet(net.n_bus, I(net.bus_id), D(net.u_rated), I(net.bus_type), D(net.u_pu), D(net.p_w),
+                  D(net.q_var), net.n_branch, I(net.br_id), I(net.br_from), I(net.br_to),
+                  I(net.br_status), D(net.br_r), D(net.br_x), D(net.br_g), D(net.br_b),
+                  net.n_app, I(net.app_id), I(net.app_node), I(net.app_status), I(net.app_type),
+                  D(net.app_p), D(net.app_q), D(net.app_g), D(net.app_b))
+    return s, keep
+
+
+def analyze(nets: Sequence[Network]) -> dict:
+    """Host-only symbolic analysis statistics of a batch (no device needed)."""
+    arr = (GaphipNet * len(nets))()
+    keep = []
+    for k, n in enumerate(nets):
+        s, kp = make_net(n)
+        arr[k] = s
+        keep.append(kp)
+    st = PlanStats()
+    _check(lib().gaphip_analyze(len(nets), arr, C.byref(st)), "analyze")
+    return st.as_dict()
+
+
+def ybus(net: Network):
+    """Y-bus in siemens assembled on the GPU: (row_ptr, col_idx, complex values)."""
+    s, keep = make_net(net)
+    cap = net.n_bus + 2 * net.n_branch
+    rp = np.zeros(net.n_bus + 1, np.int32)
+    ci = np.zeros(max(cap, 1), np.int32)
+    v = np.zeros(2 * max(cap, 1))
+    nnz = C.c_int(0)
+    _check(lib().gaphip_ybus_build(C.byref(s), _p(rp, _i), _p(ci, _i), _p(v, _d), cap, C.byref(nnz)),
+           "ybus_build")
+    k = nnz.value
+    return rp, ci[:k].copy(), v[0:2 * k:2] + 1j * v[1:2 * k:2]
+
+
+class Plan:
+    """A batch of networks solved together on the GPU (one plan, packed)."""
+
+    def __init__(self, nets: Sequence[Network], precision: int | str = FP64):
+        if isinstance(precision, str):
+            precision = {"fp64": FP64, "mixed": MIXED}[precision]
+        self.nets = list(nets)
+        self.precision = precision
+        arr = (GaphipNet * len(self.nets))()
+        self._keep = []
+        for k, n in enumerate(self.nets):
+            s, keep = make_net(n)
+            arr[k] = s
+            self._keep.append(keep)
+        h = C.c_void_p()
+        _check(lib().gaphip_plan_create(len(self.nets), arr, int(precision), C.byref(h)), "plan_create")
+        self._h = h
+        self._keep = None
+        self.bus_off = np.concatenate([[0], np.cumsum([n.n_bus for n in self.nets])]).astype(np.int64)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gaphip_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stats(self) -> dict:
+        st = PlanStats()
+        _check(lib().gaphip_plan_stats_get(self._h, C.byref(st)), "plan_stats")
+        return st.as_dict()
+
+    def set_injections(self, inst: int, p_w, q_var):
+        p = np.ascontiguousarray(p_w, np.float64)
+        q = np.ascontiguousarray(q_var, np.float64)
+        if len(p) != self.nets[inst].n_bus or len(q) != self.nets[inst].n_bus:
+            raise ValueError("injection arrays must have n_bus entries")
+        _check(lib().gaphip_plan_set_injections(self._h, inst, _p(p, _d), _p(q, _d)), "set_injections")
+
+    def set_branch_rx(self, inst: int, r, x):
+        r = np.ascontiguousarray(r, np.float64)
+        x = np.ascontiguousarray(x, np.float64)
+        if len(r) != self.nets[inst].n_branch or len(x) != self.nets[inst].n_branch:
+            raise ValueError("r / x arrays must have n_branch entries")
+        _check(lib().gaphip_plan_set_branch_rx(self._h, inst, _p(r, _d), _p(x, _d)), "set_branch_rx")
+
+    def solve(self, tolerance=1e-8, max_iterations=100, base_power=100e6, fetch_v=True):
+        G = len(self.nets)
+        o = SolveOpts(tolerance, int(max_iterations), base_power)
+        v = np.zeros(2 * int(self.bus_off[-1])) if fetch_v else None
+        it = np.zeros(G, np.int32)
+        cv = np.zeros(G, np.int32)
+        fm = np.zeros(G)
+        _check(lib().gaphip_solve(self._h, C.byref(o), _p(v, _d) if fetch_v else None, _p(it, _i),
+                                  _p(cv, _i), _p(fm, _d)), "solve")
+        out = {"iterations": it, "converged": cv.astype(bool), "final_mismatch": fm}
+        if fetch_v:
+            vc = v[0::2] + 1j * v[1::2]
+            out["v"] = [vc[self.bus_off[k]:self.bus_off[k + 1]] for k in range(G)]
+        return out
+
+    def trajectory(self, inst: int, cap: int = 256):
+        out = np.zeros(cap)
+        n = C.c_int(0)
+        _check(lib().gaphip_trajectory(self._h, inst, _p(out, _d), cap, C.byref(n)), "trajectory")
+        return out[:min(n.value, cap)].copy()
+
+    def jacobian(self, inst: int, v, base_power=100e6):
+        v = np.asarray(v, np.complex128)
+        vri = np.ascontiguousarray(np.stack([v.real, v.imag], 1).ravel())
+        nb = self.nets[inst].n_bus
+        cap = 8 * (2 * self.nets[inst].n_branch + nb) + 16
+        rp = np.zeros(2 * nb + 2, np.int32)
+        ci = np.zeros(cap, np.int32)
+        val = np.zeros(cap)
+        nv = C.c_int(0)
+        nnz = C.c_int(0)
+        _check(lib().gaphip_jacobian(self._h, inst, _p(vri, _d), base_power, C.byref(nv), _p(rp, _i),
+                                     _p(ci, _i), _p(val, _d), cap, C.byref(nnz)), "jacobian")
+        return rp[:nv.value + 1].copy(), ci[:nnz.value].copy(), val[:nnz.value].copy()
+
+    def mismatches(self, inst: int, v, base_power=100e6):
+        v = np.asarray(v, np.complex128)
+        vri = np.ascontiguousarray(np.stack([v.real, v.imag], 1).ravel())
+        cap = 2 * self.nets[inst].n_bus + 1
+        f = np.zeros(cap)
+        m = C.c_int(0)
+        _check(lib().gaphip_mismatches(self._h, inst, _p(vri, _d), base_power, _p(f, _d), cap,
+                                       C.byref(m)), "mismatches")
+        return f[:m.value].copy()
+
+    def adjoint(self, w_loss=1.0, w_dev=1.0, g_vm=None, g_va=None):
+        nb = int(self.bus_off[-1])
+        nbr = sum(n.n_branch for n in self.nets)
+        gp, gq = np.zeros(nb), np.zeros(nb)
+        gr, gx = np.zeros(max(nbr, 1)), np.zeros(max(nbr, 1))
+        loss = np.zeros(len(self.nets))
+        gvm = None if g_vm is None else np.ascontiguousarray(g_vm, np.float64)
+        gva = None if g_va is None else np.ascontiguousarray(g_va, np.float64)
+        _check(lib().gaphip_adjoint(self._h, w_loss, w_dev, None if gvm is None else _p(gvm, _d),
+                                    None if gva is None else _p(gva, _d), _p(gp, _d), _p(gq, _d),
+                                    _p(gr, _d), _p(gx, _d), _p(loss, _d)), "adjoint")
+        return {"grad_p": gp, "grad_q": gq, "grad_r": gr[:nbr], "grad_x": gx[:nbr], "loss": loss}
+
+    def profile(self, on: bool = True):
+        _check(lib().gaphip_profile_enable(self._h, int(on)), "profile_enable")
+
+    def profile_report(self) -> dict:
+        out = {}
+        for k in range(lib().gaphip_kernel_count()):
+            ms = C.c_double(0)
+            n = C.c_longlong(0)
+            b = C.c_double(0)
+            _check(lib().gaphip_profile_get(self._h, k, C.byref(ms), C.byref(n), C.byref(b)), "profile_get")
+            out[lib().gaphip_kernel_name(k).decode()] = {"ms": ms.value, "launches": n.value,
+                                                         "bytes": b.value}
+        return out
+
+
+class LU:
+    """General sparse LU on the GPU (complex CSR), the ILUSolver engine."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().gaphip_lu_create(C.byref(h)), "lu_create")
+        self._h = h
+        self.n = 0
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().gaphip_lu_destroy(self._h)
+        except Exception:
+            pass
+
+    def factorize(self, n_rows, n_cols, row_ptr, col_idx, values):
+        rp = np.ascontiguousarray(row_ptr, np.int32)
+        ci = np.ascontiguousarray(col_idx, np.int32)
+        v = np.asarray(values, np.complex128)
+        vri = np.ascontiguousarray(np.stack([v.real, v.imag], 1).ravel())
+        _check(lib().gaphip_lu_factorize(self._h, int(n_rows), int(n_cols), _p(rp, _i), _p(ci, _i),
+                                         _p(vri, _d), len(ci)), "lu_factorize")
+        self.n = int(n_rows)
+
+    def solve(self, rhs):
+        b = np.asarray(rhs, np.complex128)
+        bri = np.ascontiguousarray(np.stack([b.real, b.imag], 1).ravel())
+        x = np.zeros(2 * len(b))
+        _check(lib().gaphip_lu_solve(self._h, _p(bri, _d), _p(x, _d), len(b)), "lu_solve")
+        return x[0::2] + 1j * x[1::2]
+
+    @property
+    def factorized(self) -> bool:
+        return bool(lib().gaphip_lu_is_factorized(self._h))

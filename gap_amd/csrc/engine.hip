@@ -1,0 +1,1473 @@
+// Device engine and C ABI of the gfx950 Newton-Raphson power-flow backend.
+//
+// A plan packs a batch of networks end to end on the device: buses, Y-bus
+// slots, Newton equations/variables, Jacobian slots and multifrontal fronts
+// all carry global indices, so every kernel launch serves the whole batch and
+// the level schedule of the fronts is the union of the per-network schedules.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gap_hip.h"
+#include "kernels.hpp"
+#include "symbolic.hpp"
+
+namespace gaphip {
+
+thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, std::string const& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw Error(GAPHIP_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(DBuf const&) = delete;
+    DBuf& operator=(DBuf const&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        n = count;
+        if (count) HIPCHK(hipMalloc(&p, count * sizeof(T)));
+    }
+    void upload(std::vector<T> const& h, hipStream_t s) {
+        alloc(h.size());
+        if (!h.empty()) HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+inline int blocks_for(long long n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
+
+// Kernel ids of the timing hook: one per kernel (front kernels per size class).
+enum Kid { K_YBUS = 0, K_FLAT, K_MISMATCH, K_CHECK, K_JAC, K_UPDATE, K_FACTOR, K_FWD = K_FACTOR + 5, K_BWD = K_FWD + 5,
+           K_COUNT = K_BWD + 5 };
+static char const* const kKernelNames[K_COUNT] = {
+    "k_ybus", "k_flat", "k_mismatch", "k_check", "k_jacobian", "k_update",
+    "k_front_factor_lds<16>", "k_front_factor_lds<32>", "k_front_factor_lds<64>", "k_front_factor_lds<128>",
+    "k_big_factor (assemble+panel+trsm+gemm)", "k_front_fwd_lds<16>", "k_front_fwd_lds<32>", "k_front_fwd_lds<64>",
+    "k_front_fwd_lds<128>", "k_front_fwd", "k_front_bwd_lds<16>", "k_front_bwd_lds<32>", "k_front_bwd_lds<64>",
+    "k_front_bwd_lds<128>", "k_front_bwd"};
+
+Net copy_net(gaphip_net const& in) {
+    if (in.n_bus <= 0) throw Error(GAPHIP_E_ARG, "network has no buses");
+    if (in.n_branch < 0 || in.n_app < 0) throw Error(GAPHIP_E_ARG, "negative component count");
+    auto need = [](void const* p, int n, char const* what) {
+        if (n > 0 && !p) throw Error(GAPHIP_E_ARG, std::string("missing array ") + what);
+    };
+    need(in.bus_id, in.n_bus, "bus_id");
+    need(in.u_rated, in.n_bus, "u_rated");
+    need(in.bus_type, in.n_bus, "bus_type");
+    need(in.u_pu, in.n_bus, "u_pu");
+    need(in.p_w, in.n_bus, "p_w");
+    need(in.q_var, in.n_bus, "q_var");
+    need(in.br_id, in.n_branch, "br_id");
+    need(in.br_from, in.n_branch, "br_from");
+    need(in.br_to, in.n_branch, "br_to");
+    need(in.br_status, in.n_branch, "br_status");
+    need(in.br_r, in.n_branch, "br_r");
+    need(in.br_x, in.n_branch, "br_x");
+    need(in.app_id, in.n_app, "app_id");
+    need(in.app_node, in.n_app, "app_node");
+    need(in.app_status, in.n_app, "app_status");
+    need(in.app_type, in.n_app, "app_type");
+    need(in.app_p, in.n_app, "app_p");
+    need(in.app_q, in.n_app, "app_q");
+    need(in.app_g, in.n_app, "app_g");
+    need(in.app_b, in.n_app, "app_b");
+    Net n;
+    auto cp = [](auto const* p, int k, auto& v) { v.assign(p, p + k); };
+    cp(in.bus_id, in.n_bus, n.bus_id);
+    cp(in.u_rated, in.n_bus, n.u_rated);
+    cp(in.bus_type, in.n_bus, n.bus_type);
+    cp(in.u_pu, in.n_bus, n.u_pu);
+    cp(in.p_w, in.n_bus, n.p_w);
+    cp(in.q_var, in.n_bus, n.q_var);
+    cp(in.br_id, in.n_branch, n.br_id);
+    cp(in.br_from, in.n_branch, n.br_from);
+    cp(in.br_to, in.n_branch, n.br_to);
+    cp(in.br_status, in.n_branch, n.br_status);
+    cp(in.br_r, in.n_branch, n.br_r);
+    cp(in.br_x, in.n_branch, n.br_x);
+    if (in.br_g) cp(in.br_g, in.n_branch, n.br_g); else n.br_g.assign(in.n_branch, 0.0);
+    if (in.br_b) cp(in.br_b, in.n_branch, n.br_b); else n.br_b.assign(in.n_branch, 0.0);
+    if (in.n_app > 0) {
+        cp(in.app_id, in.n_app, n.app_id);
+        cp(in.app_node, in.n_app, n.app_node);
+        cp(in.app_status, in.n_app, n.app_status);
+        cp(in.app_type, in.n_app, n.app_type);
+        cp(in.app_p, in.n_app, n.app_p);
+        cp(in.app_q, in.n_app, n.app_q);
+        cp(in.app_g, in.n_app, n.app_g);
+        cp(in.app_b, in.n_app, n.app_b);
+    }
+    return n;
+}
+
+// Builds the LU input of a Newton system: nodes are the non-slack buses, a PQ
+// node carries (theta <-> P, |V| <-> Q), a PV node (theta <-> P); entries are
+// the structural Jacobian slots in reference CSR order.
+LUInput nr_lu_input(Net const& net, YPlan const& y, NRPlan const& p, std::vector<int>& node_bus) {
+    int const n = net.n_bus();
+    std::vector<int> node_of(n, -1);
+    node_bus.clear();
+    for (int i = 0; i < n; ++i)
+        if (net.bus_type[i] != kSlack) {
+            node_of[i] = static_cast<int>(node_bus.size());
+            node_bus.push_back(i);
+        }
+    LUInput in;
+    in.n_nodes = static_cast<int>(node_bus.size());
+    in.w.resize(in.n_nodes);
+    for (int k = 0; k < in.n_nodes; ++k) in.w[k] = net.bus_type[node_bus[k]] == kPQ ? 2 : 1;
+    in.adj_ptr.assign(in.n_nodes + 1, 0);
+    std::vector<int> mark(in.n_nodes, -1);
+    for (int k = 0; k < in.n_nodes; ++k) {
+        int i = node_bus[k];
+        mark[k] = k;
+        for (int q = y.rp[i]; q < y.rp[i + 1]; ++q) {
+            int u = node_of[y.ci[q]];
+            if (u < 0 || mark[u] == k) continue;
+            mark[u] = k;
+            in.adj.push_back(u);
+        }
+        in.adj_ptr[k + 1] = static_cast<int>(in.adj.size());
+    }
+    // Y is pattern-symmetric (both directions of every branch), so adj is too.
+    std::vector<int> eq_key(p.n_var), var_key(p.n_var);
+    for (int i = 0; i < n; ++i) {
+        int nd = node_of[i];
+        if (p.peq[i] >= 0) eq_key[p.peq[i]] = 2 * nd;
+        if (p.qeq[i] >= 0) eq_key[p.qeq[i]] = 2 * nd + 1;
+        if (p.ang[i] >= 0 && nd >= 0) var_key[p.ang[i]] = 2 * nd;
+        if (p.mag[i] >= 0) var_key[p.mag[i]] = 2 * nd + 1;
+    }
+    int const nnz = p.jrp[p.n_var];
+    in.ent_row.resize(nnz);
+    in.ent_col.resize(nnz);
+    for (int r = 0; r < p.n_var; ++r)
+        for (int q = p.jrp[r]; q < p.jrp[r + 1]; ++q) {
+            in.ent_row[q] = eq_key[r];
+            in.ent_col[q] = var_key[p.jci[q]];
+        }
+    return in;
+}
+
+// Device copy of a (possibly batched) multifrontal plan.
+// Front size classes: LDS-resident kernels for fronts of up to 16 / 32 / 64 /
+// 128 slots, the global-memory kernel above that.
+constexpr int kNumClasses = 5;
+constexpr int kClassMax[kNumClasses - 1] = {16, 32, 64, 128};
+inline int front_class(int m) {
+    for (int c = 0; c < kNumClasses - 1; ++c)
+        if (m <= kClassMax[c]) return c;
+    return kNumClasses - 1;
+}
+
+struct DevFronts {
+    int n_fronts = 0, n_levels = 0;
+    std::vector<int> level_ptr;  // host copy of the level schedule
+    // fronts of level l and class c: level_front[cls_ptr[l*K+c] .. cls_ptr[l*K+c+1])
+    std::vector<int> cls_ptr;
+    std::vector<long long> cls_arrow, cls_factor, cls_slots;  // per (level, class) totals
+    std::vector<int> cls_mmax, cls_pmax;                      // per (level, class) largest m / p
+    DBuf<int> bigbase, arow, crow, crow_c, crow_a;            // row maps of fronts above 128 slots
+    DBuf<int> level_front, f_m, f_p, f_slot_ptr, f_child_ptr, child, f_rel_off, rel, f_arrow_ptr, arrow_src,
+        arrow_dst, f_inst, slot_row, slot_col;
+    DBuf<long long> f_off, f_voff;
+    long long front_entries = 0, vec_entries = 0, factor_entries = 0;
+    int max_front = 0;
+
+    BigMaps big() const { return BigMaps{bigbase.p, arow.p, crow.p, crow_c.p, crow_a.p}; }
+
+    FrontArgs args(int const* done, int* fail) const {
+        FrontArgs a;
+        a.level_front = level_front.p;
+        a.f_m = f_m.p;
+        a.f_p = f_p.p;
+        a.f_off = f_off.p;
+        a.f_voff = f_voff.p;
+        a.f_slot_ptr = f_slot_ptr.p;
+        a.f_child_ptr = f_child_ptr.p;
+        a.child = child.p;
+        a.f_rel_off = f_rel_off.p;
+        a.rel = rel.p;
+        a.f_arrow_ptr = f_arrow_ptr.p;
+        a.arrow_src = arrow_src.p;
+        a.arrow_dst = arrow_dst.p;
+        a.f_inst = f_inst.p;
+        a.done = done;
+        a.fail = fail;
+        a.slot_row = slot_row.p;
+        a.slot_col = slot_col.p;
+        return a;
+    }
+};
+
+// Concatenate per-instance LU plans.  key_row / key_col map a slot key of
+// instance g to its global right-hand-side / solution index; ent_base[g] is
+// the offset of instance g's entries in the global value array.
+template <typename KeyRow, typename KeyCol>
+void pack_fronts(std::vector<LUPlan const*> const& plans, std::vector<long long> const& ent_base,
+                 KeyRow key_row, KeyCol key_col, long long n_index, long long n_values, DevFronts& d,
+                 hipStream_t s) {
+    int const G = static_cast<int>(plans.size());
+    std::vector<int> f_m, f_p, f_slot_ptr{0}, f_child_ptr{0}, child, f_rel_off{0}, rel, f_arrow_ptr{0},
+        arrow_src, arrow_dst, f_inst, slot_row, slot_col, f_level;
+    std::vector<long long> f_off{0}, f_voff{0};
+    int nlev = 0, fbase = 0;
+    d.max_front = 0;
+    d.factor_entries = 0;
+    for (int g = 0; g < G; ++g) {
+        LUPlan const& P = *plans[g];
+        int const slot0 = f_slot_ptr.back(), child0 = f_child_ptr.back(), rel0 = f_rel_off.back(),
+                  arr0 = f_arrow_ptr.back();
+        long long const off0 = f_off.back(), voff0 = f_voff.back();
+        for (int s2 = 0; s2 < P.n_fronts; ++s2) {
+            f_m.push_back(P.f_m[s2]);
+            f_p.push_back(P.f_p[s2]);
+            f_slot_ptr.push_back(slot0 + P.f_slot_ptr[s2 + 1]);
+            f_child_ptr.push_back(child0 + P.f_child_ptr[s2 + 1]);
+            f_rel_off.push_back(rel0 + P.f_rel_off[s2 + 1]);
+            f_arrow_ptr.push_back(arr0 + P.f_arrow_ptr[s2 + 1]);
+            f_off.push_back(off0 + P.f_off[s2 + 1]);
+            f_voff.push_back(voff0 + P.f_voff[s2 + 1]);
+            f_inst.push_back(g);
+            f_level.push_back(P.f_level[s2]);
+        }
+        for (int c : P.child) child.push_back(fbase + c);
+        rel.insert(rel.end(), P.rel.begin(), P.rel.end());
+        for (int q = 0; q < static_cast<int>(P.arrow_src.size()); ++q) {
+            arrow_src.push_back(static_cast<int>(ent_base[g] + P.arrow_src[q]));
+            arrow_dst.push_back(P.arrow_dst[q]);
+        }
+        for (int key : P.slot_key) {
+            slot_row.push_back(key_row(g, key));
+            slot_col.push_back(key_col(g, key));
+        }
+        nlev = std::max(nlev, P.n_levels);
+        fbase += P.n_fronts;
+        d.max_front = std::max(d.max_front, P.max_front);
+        d.factor_entries += P.factor_entries;
+    }
+    d.n_fronts = fbase;
+    d.n_levels = nlev;
+    int const K = kNumClasses;
+    d.cls_ptr.assign(static_cast<size_t>(nlev) * K + 1, 0);
+    for (int f = 0; f < fbase; ++f) {
+        if (f_m[f] > 65535) throw Error(GAPHIP_E_ARG, "front larger than 65535 slots");
+        d.cls_ptr[static_cast<size_t>(f_level[f]) * K + front_class(f_m[f]) + 1]++;
+    }
+    for (size_t b = 0; b + 1 < d.cls_ptr.size(); ++b) d.cls_ptr[b + 1] += d.cls_ptr[b];
+    d.level_ptr.assign(nlev + 1, 0);
+    for (int l = 0; l <= nlev; ++l) d.level_ptr[l] = d.cls_ptr[static_cast<size_t>(l) * K];
+    std::vector<int> level_front(fbase);
+    d.cls_mmax.assign(static_cast<size_t>(nlev) * K, 0);
+    d.cls_pmax.assign(static_cast<size_t>(nlev) * K, 0);
+    d.cls_arrow.assign(static_cast<size_t>(nlev) * K, 0);
+    d.cls_factor.assign(static_cast<size_t>(nlev) * K, 0);
+    d.cls_slots.assign(static_cast<size_t>(nlev) * K, 0);
+    {
+        std::vector<int> fp(d.cls_ptr.begin(), d.cls_ptr.end() - 1);
+        for (int f = 0; f < fbase; ++f) {
+            size_t b = static_cast<size_t>(f_level[f]) * K + front_class(f_m[f]);
+            level_front[fp[b]++] = f;
+            long long m = f_m[f], pp = f_p[f];
+            d.cls_arrow[b] += f_arrow_ptr[f + 1] - f_arrow_ptr[f];
+            d.cls_factor[b] += pp * m + (m - pp) * pp;
+            d.cls_slots[b] += m;
+            d.cls_mmax[b] = std::max<int>(d.cls_mmax[b], static_cast<int>(m));
+            d.cls_pmax[b] = std::max<int>(d.cls_pmax[b], static_cast<int>(pp));
+        }
+    }
+    // arrowhead entries of every front in row-major order of their targets
+    for (int f = 0; f < fbase; ++f) {
+        int const b0 = f_arrow_ptr[f], b1 = f_arrow_ptr[f + 1];
+        std::vector<std::pair<int, int>> e;
+        e.reserve(b1 - b0);
+        for (int q = b0; q < b1; ++q) e.emplace_back(arrow_dst[q], arrow_src[q]);
+        std::sort(e.begin(), e.end());
+        for (int q = b0; q < b1; ++q) {
+            arrow_dst[q] = e[q - b0].first;
+            arrow_src[q] = e[q - b0].second;
+        }
+    }
+    // per-row maps of the fronts handled by the blocked kernels
+    {
+        std::vector<int> bigbase(fbase, -1), arow, crow, crow_c, crow_a;
+        int rows = 0;
+        for (int f = 0; f < fbase; ++f) {
+            if (front_class(f_m[f]) != kNumClasses - 1) continue;
+            int const m = f_m[f];
+            bigbase[f] = rows;
+            rows += m + 1;
+            std::vector<int> na(m + 1, 0), nc(m + 1, 0);
+            for (int q = f_arrow_ptr[f]; q < f_arrow_ptr[f + 1]; ++q) na[(arrow_dst[q] >> 16) + 1]++;
+            for (int cq = f_child_ptr[f]; cq < f_child_ptr[f + 1]; ++cq) {
+                int c = child[cq];
+                for (int t = 0; t < f_m[c] - f_p[c]; ++t) nc[rel[f_rel_off[c] + t] + 1]++;
+            }
+            for (int r = 0; r < m; ++r) {
+                na[r + 1] += na[r];
+                nc[r + 1] += nc[r];
+            }
+            int const cbase = static_cast<int>(crow_c.size());
+            for (int r = 0; r <= m; ++r) {
+                arow.push_back(na[r]);
+                crow.push_back(cbase + nc[r]);
+            }
+            crow_c.resize(cbase + nc[m]);
+            crow_a.resize(cbase + nc[m]);
+            std::vector<int> fill(nc.begin(), nc.end() - 1);
+            for (int cq = f_child_ptr[f]; cq < f_child_ptr[f + 1]; ++cq) {
+                int c = child[cq];
+                for (int t = 0; t < f_m[c] - f_p[c]; ++t) {
+                    int r = rel[f_rel_off[c] + t];
+                    int at = cbase + fill[r]++;
+                    crow_c[at] = c;
+                    crow_a[at] = t;
+                }
+            }
+        }
+        if (arow.empty()) {
+            arow.push_back(0);
+            crow.push_back(0);
+        }
+        if (crow_c.empty()) {
+            crow_c.push_back(0);
+            crow_a.push_back(0);
+        }
+        d.bigbase.upload(bigbase, s);
+        d.arow.upload(arow, s);
+        d.crow.upload(crow, s);
+        d.crow_c.upload(crow_c, s);
+        d.crow_a.upload(crow_a, s);
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    d.front_entries = f_off.back();
+    d.vec_entries = f_voff.back();
+    // every index the front kernels dereference is checked here, on the host
+    {
+        long long const n_slots = static_cast<long long>(slot_row.size());
+        for (int f = 0; f < fbase; ++f) {
+            long long m = f_m[f], p = f_p[f];
+            if (m <= 0 || p <= 0 || p > m) throw Error(GAPHIP_E_ARG, "front with invalid size");
+            if (f_off[f + 1] - f_off[f] != m * m || f_voff[f + 1] - f_voff[f] != m)
+                throw Error(GAPHIP_E_ARG, "front offsets inconsistent");
+            if (f_slot_ptr[f + 1] - f_slot_ptr[f] != m || f_slot_ptr[f + 1] > n_slots)
+                throw Error(GAPHIP_E_ARG, "front slots inconsistent");
+            for (int q = f_arrow_ptr[f]; q < f_arrow_ptr[f + 1]; ++q)
+                if (arrow_dst[q] < 0 || (arrow_dst[q] >> 16) >= m || (arrow_dst[q] & 0xffff) >= m)
+                    throw Error(GAPHIP_E_ARG, "arrowhead map out of range");
+            for (int q = f_child_ptr[f]; q < f_child_ptr[f + 1]; ++q) {
+                int c = child[q];
+                if (c < 0 || c >= f) throw Error(GAPHIP_E_ARG, "child must precede its parent");
+                int nu = f_m[c] - f_p[c];
+                if (f_rel_off[c + 1] - f_rel_off[c] != nu) throw Error(GAPHIP_E_ARG, "extend-add map size");
+                for (int t = 0; t < nu; ++t)
+                    if (rel[f_rel_off[c] + t] < 0 || rel[f_rel_off[c] + t] >= m)
+                        throw Error(GAPHIP_E_ARG, "extend-add map out of range");
+            }
+        }
+        for (long long q = 0; q < n_slots; ++q)
+            if (slot_row[q] < 0 || slot_col[q] < 0 || slot_row[q] >= n_index || slot_col[q] >= n_index)
+                throw Error(GAPHIP_E_ARG, "slot without equation/variable");
+        for (int src : arrow_src)
+            if (src < 0 || src >= n_values) throw Error(GAPHIP_E_ARG, "arrowhead source out of range");
+        if (static_cast<long long>(f_m.size()) != fbase || static_cast<int>(level_front.size()) != fbase)
+            throw Error(GAPHIP_E_ARG, "front count mismatch");
+    }
+    d.level_front.upload(level_front, s);
+    d.f_m.upload(f_m, s);
+    d.f_p.upload(f_p, s);
+    d.f_slot_ptr.upload(f_slot_ptr, s);
+    d.f_child_ptr.upload(f_child_ptr, s);
+    d.child.upload(child, s);
+    d.f_rel_off.upload(f_rel_off, s);
+    d.rel.upload(rel, s);
+    d.f_arrow_ptr.upload(f_arrow_ptr, s);
+    d.arrow_src.upload(arrow_src, s);
+    d.arrow_dst.upload(arrow_dst, s);
+    d.f_inst.upload(f_inst, s);
+    d.slot_row.upload(slot_row, s);
+    d.slot_col.upload(slot_col, s);
+    d.f_off.upload(f_off, s);
+    d.f_voff.upload(f_voff, s);
+    HIPCHK(hipStreamSynchronize(s));  // host vectors die at return
+}
+
+// Runs factorisation / solves over all levels of a packed plan, one launch per
+// non-empty (level, size class).  `hook(kid, bytes, fn)` wraps every launch.
+template <typename T>
+struct FrontRunner {
+    static constexpr int kPanel = 32;
+    // blocked partial LU of the fronts [beg, beg+nb) of one (level, class > 128) group
+    static void big_factor(DevFronts const& d, FrontArgs const& a, int beg, int nb, int mmax, int pmax,
+                           T const* jv, T* front, hipStream_t s) {
+        hipLaunchKernelGGL(k_big_assemble<T>, dim3(mmax, nb), dim3(256), 0, s, a, d.big(), beg, jv, front);
+        for (int k0 = 0; k0 < pmax; k0 += kPanel) {
+            hipLaunchKernelGGL(k_big_panel<T>, dim3(nb), dim3(256), 0, s, a, beg, k0, kPanel, front);
+            int const rest = mmax - std::min(k0 + kPanel, pmax);
+            if (rest <= 0) continue;
+            hipLaunchKernelGGL(k_big_trsm<T>, dim3(blocks_for(rest, 256), nb), dim3(256), 0, s, a, beg, k0, kPanel,
+                               front);
+            int const tiles = blocks_for(rest, 64);
+            hipLaunchKernelGGL(k_big_gemm<T>, dim3(tiles, tiles, nb), dim3(256), 0, s, a, beg, k0, kPanel, front);
+        }
+    }
+    template <typename Hook>
+    static void factor(DevFronts const& d, FrontArgs const& a, T const* jv, T* front, hipStream_t s, int kid,
+                       Hook&& hook) {
+        double const w = sizeof(T);
+        for (int l = 0; l < d.n_levels; ++l)
+            for (int c = 0; c < kNumClasses; ++c) {
+                size_t b = static_cast<size_t>(l) * kNumClasses + c;
+                int const beg = d.cls_ptr[b], nb = d.cls_ptr[b + 1] - beg;
+                if (!nb) continue;
+                double bytes = (w + 4) * d.cls_arrow[b] + w * d.cls_factor[b];
+                hook(kid + c, bytes, [&] {
+                    switch (c) {
+                        case 0: hipLaunchKernelGGL((k_front_factor_lds<T, 16, 64>), dim3(nb), dim3(64), 0, s, a, beg, jv, front); break;
+                        case 1: hipLaunchKernelGGL((k_front_factor_lds<T, 32, 64>), dim3(nb), dim3(64), 0, s, a, beg, jv, front); break;
+                        case 2: hipLaunchKernelGGL((k_front_factor_lds<T, 64, 64>), dim3(nb), dim3(64), 0, s, a, beg, jv, front); break;
+                        case 3: hipLaunchKernelGGL((k_front_factor_lds<T, 128, 128>), dim3(nb), dim3(128), 0, s, a, beg, jv, front); break;
+                        default: big_factor(d, a, beg, nb, d.cls_mmax[b], d.cls_pmax[b], jv, front, s);
+                    }
+                });
+            }
+    }
+    template <typename Hook>
+    static void forward(DevFronts const& d, FrontArgs const& a, T const* front, T* vec, double const* rhs,
+                        double sign, hipStream_t s, int kid, Hook&& hook) {
+        double const w = sizeof(T);
+        for (int l = 0; l < d.n_levels; ++l)
+            for (int c = 0; c < kNumClasses; ++c) {
+                size_t b = static_cast<size_t>(l) * kNumClasses + c;
+                int const beg = d.cls_ptr[b], nb = d.cls_ptr[b + 1] - beg;
+                if (!nb) continue;
+                double bytes = w * d.cls_factor[b] / 2 + (2 * w + 12) * d.cls_slots[b];
+                hook(kid + c, bytes, [&] {
+                    switch (c) {
+                        case 0: hipLaunchKernelGGL((k_front_fwd_lds<T, 16, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, rhs, sign); break;
+                        case 1: hipLaunchKernelGGL((k_front_fwd_lds<T, 32, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, rhs, sign); break;
+                        case 2: hipLaunchKernelGGL((k_front_fwd_lds<T, 64, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, rhs, sign); break;
+                        case 3: hipLaunchKernelGGL((k_front_fwd_lds<T, 128, 128>), dim3(nb), dim3(128), 0, s, a, beg, front, vec, rhs, sign); break;
+                        default: hipLaunchKernelGGL(k_front_fwd<T>, dim3(nb), dim3(256), 0, s, a, beg, front, vec, rhs, sign);
+                    }
+                });
+            }
+    }
+    template <typename Hook>
+    static void backward(DevFronts const& d, FrontArgs const& a, T const* front, T* vec, double* sol,
+                         hipStream_t s, int kid, Hook&& hook) {
+        double const w = sizeof(T);
+        for (int l = d.n_levels - 1; l >= 0; --l)
+            for (int c = 0; c < kNumClasses; ++c) {
+                size_t b = static_cast<size_t>(l) * kNumClasses + c;
+                int const beg = d.cls_ptr[b], nb = d.cls_ptr[b + 1] - beg;
+                if (!nb) continue;
+                double bytes = w * d.cls_factor[b] / 2 + (2 * w + 12) * d.cls_slots[b];
+                hook(kid + c, bytes, [&] {
+                    switch (c) {
+                        case 0: hipLaunchKernelGGL((k_front_bwd_lds<T, 16, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, sol); break;
+                        case 1: hipLaunchKernelGGL((k_front_bwd_lds<T, 32, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, sol); break;
+                        case 2: hipLaunchKernelGGL((k_front_bwd_lds<T, 64, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, sol); break;
+                        case 3: hipLaunchKernelGGL((k_front_bwd_lds<T, 128, 128>), dim3(nb), dim3(128), 0, s, a, beg, front, vec, sol); break;
+                        default: hipLaunchKernelGGL(k_front_bwd<T>, dim3(nb), dim3(256), 0, s, a, beg, front, vec, sol);
+                    }
+                });
+            }
+    }
+};
+
+// Launch hook that runs the launch as is.
+struct PlainHook {
+    template <typename F>
+    void operator()(int, double, F&& fn) const {
+        fn();
+    }
+};
+
+// --------------------------------------------------------------- the plan
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        int kid;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double ms[K_COUNT] = {};
+    long long launches[K_COUNT] = {};
+    double bytes[K_COUNT] = {};
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    void reset() {
+        for (int k = 0; k < K_COUNT; ++k) ms[k] = 0, launches[k] = 0, bytes[k] = 0;
+    }
+    void collect() {
+        for (auto& r : recs) {
+            HIPCHK(hipEventSynchronize(r.b));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+            ms[r.kid] += t;
+            launches[r.kid]++;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        recs.clear();
+    }
+    ~Profiler() {
+        for (auto& r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace gaphip
+
+struct gaphip_plan {
+    int n_inst = 0;
+    int precision = GAPHIP_FP64;
+    std::vector<gaphip::Net> nets;
+    std::vector<gaphip::YPlan> yp;
+    std::vector<gaphip::NRPlan> np;
+    std::vector<gaphip::LUPlan> lp;
+    std::vector<std::vector<int>> node_bus;
+    std::vector<int> bus_off, br_off, app_off, y_off, var_off, j_off, item_off;
+    long long n_bus = 0, n_br = 0, n_app = 0, y_nnz = 0, n_var = 0, j_nnz = 0, n_items = 0;
+    double analysis_ms = 0;
+    hipStream_t stream = nullptr;
+
+    // device data
+    gaphip::DBuf<int> bus_inst, ang, mag, peq, qeq, y_rp, y_ci, y_seg, y_code, y_slot_inst, it_i, it_j, it_y,
+        it_pos, done, iters, lu_fail, active;
+    gaphip::DBuf<double> flat, br_r, br_x, br_g, br_b, app_g, app_b, zb, f, dx, final_mis, jv64;
+    gaphip::DBuf<float> jv32;
+    gaphip::DBuf<double2> v, vpol, s, sspec, y_pu, y_given;
+    std::vector<int> y_given_mode;  // per instance: 0 assemble, 1 siemens given, 2 per-unit given
+    gaphip::DBuf<unsigned long long> maxf;
+    gaphip::DBuf<double> front64, vec64;
+    gaphip::DBuf<float> front32, vec32;
+    gaphip::DevFronts fr;
+
+    // host state
+    bool sspec_dirty = true;
+    bool rx_dirty = false;
+    double sspec_base = -1.0;
+    int last_iters_run = 0;
+    int maxf_rows = 0;
+    std::vector<unsigned long long> maxf_host;
+    bool have_solution = false;
+    double last_base = 100e6;
+    gaphip::Profiler prof;
+    int* pinned_active = nullptr;
+    hipEvent_t ev = nullptr;
+
+    ~gaphip_plan() {
+        if (pinned_active) (void)hipHostFree(pinned_active);
+        if (ev) (void)hipEventDestroy(ev);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace gaphip {
+
+void analyze(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precision) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (n_inst <= 0 || !nets) throw Error(GAPHIP_E_ARG, "plan needs at least one network");
+    if (precision != GAPHIP_FP64 && precision != GAPHIP_MIXED) throw Error(GAPHIP_E_ARG, "unknown precision");
+    P.n_inst = n_inst;
+    P.precision = precision;
+    P.nets.resize(n_inst);
+    P.yp.resize(n_inst);
+    P.np.resize(n_inst);
+    P.lp.resize(n_inst);
+    P.node_bus.resize(n_inst);
+    for (int g = 0; g < n_inst; ++g) P.nets[g] = copy_net(nets[g]);
+    // symbolic analysis, instances spread over host threads
+    std::vector<std::string> errs(n_inst);
+    std::vector<int> bad(n_inst, 0);
+    auto work = [&](int g) {
+        try {
+            P.yp[g] = plan_ybus(P.nets[g]);
+            if (!plan_nr(P.nets[g], P.yp[g], P.np[g], errs[g])) {
+                bad[g] = 1;
+                return;
+            }
+            LUInput in = nr_lu_input(P.nets[g], P.yp[g], P.np[g], P.node_bus[g]);
+            P.lp[g] = plan_lu(in);
+        } catch (std::exception const& e) {
+            errs[g] = e.what();
+            bad[g] = 1;
+        }
+    };
+    int nthr = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    nthr = std::min(nthr, n_inst);
+    if (nthr <= 1) {
+        for (int g = 0; g < n_inst; ++g) work(g);
+    } else {
+        std::vector<std::thread> th;
+        std::atomic<int> next{0};
+        for (int t = 0; t < nthr; ++t)
+            th.emplace_back([&]() {
+                for (int g = next++; g < n_inst; g = next++) work(g);
+            });
+        for (auto& x : th) x.join();
+    }
+    for (int g = 0; g < n_inst; ++g)
+        if (bad[g]) throw Error(GAPHIP_E_ARG, "network " + std::to_string(g) + ": " + errs[g]);
+
+    // offsets
+    P.bus_off.assign(n_inst + 1, 0);
+    P.br_off.assign(n_inst + 1, 0);
+    P.app_off.assign(n_inst + 1, 0);
+    P.y_off.assign(n_inst + 1, 0);
+    P.var_off.assign(n_inst + 1, 0);
+    P.j_off.assign(n_inst + 1, 0);
+    P.item_off.assign(n_inst + 1, 0);
+    for (int g = 0; g < n_inst; ++g) {
+        auto const& N = P.nets[g];
+        P.bus_off[g + 1] = P.bus_off[g] + N.n_bus();
+        P.br_off[g + 1] = P.br_off[g] + N.n_branch();
+        P.app_off[g + 1] = P.app_off[g] + N.n_app();
+        P.y_off[g + 1] = P.y_off[g] + static_cast<int>(P.yp[g].ci.size());
+        P.var_off[g + 1] = P.var_off[g] + P.np[g].n_var;
+        P.j_off[g + 1] = P.j_off[g] + static_cast<int>(P.np[g].jci.size());
+        P.item_off[g + 1] = P.item_off[g] + static_cast<int>(P.np[g].it_i.size());
+    }
+    P.n_bus = P.bus_off[n_inst];
+    P.n_br = P.br_off[n_inst];
+    P.n_app = P.app_off[n_inst];
+    P.y_nnz = P.y_off[n_inst];
+    P.n_var = P.var_off[n_inst];
+    P.j_nnz = P.j_off[n_inst];
+    P.n_items = P.item_off[n_inst];
+
+    P.analysis_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precision) {
+    auto t0 = std::chrono::steady_clock::now();
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+        throw Error(GAPHIP_E_DEVICE, "no HIP device visible");
+    analyze(P, n_inst, nets, precision);
+    HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&P.pinned_active), sizeof(int)));
+    HIPCHK(hipEventCreateWithFlags(&P.ev, hipEventDisableTiming));
+    hipStream_t s = P.stream;
+
+    std::vector<int> bus_inst(P.n_bus), ang(P.n_bus), mag(P.n_bus), peq(P.n_bus), qeq(P.n_bus);
+    std::vector<double> flat(P.n_bus), br_r(P.n_br), br_x(P.n_br), br_g(P.n_br), br_b(P.n_br), app_g(P.n_app),
+        app_b(P.n_app);
+    std::vector<int> y_rp(P.n_bus + 1), y_ci(P.y_nnz), y_seg, y_code, y_slot_inst(P.y_nnz);
+    std::vector<int> it_i(P.n_items), it_j(P.n_items), it_y(P.n_items), it_pos(4 * P.n_items);
+    y_seg.reserve(P.y_nnz + 1);
+    y_seg.push_back(0);
+    for (int g = 0; g < n_inst; ++g) {
+        auto const& N = P.nets[g];
+        auto const& Y = P.yp[g];
+        auto const& R = P.np[g];
+        int const b0 = P.bus_off[g], v0 = P.var_off[g], y0 = P.y_off[g];
+        for (int i = 0; i < N.n_bus(); ++i) {
+            bus_inst[b0 + i] = g;
+            ang[b0 + i] = R.ang[i] >= 0 ? v0 + R.ang[i] : -1;
+            mag[b0 + i] = R.mag[i] >= 0 ? v0 + R.mag[i] : -1;
+            peq[b0 + i] = R.peq[i] >= 0 ? v0 + R.peq[i] : -1;
+            qeq[b0 + i] = R.qeq[i] >= 0 ? v0 + R.qeq[i] : -1;
+            flat[b0 + i] = (N.bus_type[i] == kSlack || N.bus_type[i] == kPV) ? N.u_pu[i] : 0.98;
+            y_rp[b0 + i] = y0 + Y.rp[i];
+        }
+        for (int k = 0; k < N.n_branch(); ++k) {
+            br_r[P.br_off[g] + k] = N.br_r[k];
+            br_x[P.br_off[g] + k] = N.br_x[k];
+            br_g[P.br_off[g] + k] = N.br_g[k];
+            br_b[P.br_off[g] + k] = N.br_b[k];
+        }
+        for (int a = 0; a < N.n_app(); ++a) {
+            app_g[P.app_off[g] + a] = N.app_g[a];
+            app_b[P.app_off[g] + a] = N.app_b[a];
+        }
+        int const cbase = static_cast<int>(y_code.size());
+        for (int c : Y.code) {
+            int kind = c & 3, idx = c >> 2;
+            idx += kind == kContribShunt ? P.app_off[g] : P.br_off[g];
+            y_code.push_back((idx << 2) | kind);
+        }
+        for (int k = 0; k < static_cast<int>(Y.ci.size()); ++k) {
+            y_ci[y0 + k] = b0 + Y.ci[k];
+            y_seg.push_back(cbase + Y.seg[k + 1]);
+            y_slot_inst[y0 + k] = g;
+        }
+        int const t0i = P.item_off[g];
+        for (int t = 0; t < static_cast<int>(R.it_i.size()); ++t) {
+            it_i[t0i + t] = b0 + R.it_i[t];
+            it_j[t0i + t] = b0 + R.it_j[t];
+            it_y[t0i + t] = y0 + R.it_y[t];
+            for (int q = 0; q < 4; ++q) {
+                int pp = R.it_pos[4 * t + q];
+                it_pos[4 * (t0i + t) + q] = pp >= 0 ? P.j_off[g] + pp : -1;
+            }
+        }
+    }
+    y_rp[P.n_bus] = static_cast<int>(P.y_nnz);
+    // bounds of every gathered index, checked before anything reaches the device
+    for (long long i = 0; i < P.n_bus; ++i) {
+        if (ang[i] >= P.n_var || mag[i] >= P.n_var || peq[i] >= P.n_var || qeq[i] >= P.n_var)
+            throw Error(GAPHIP_E_ARG, "Newton index out of range");
+        if (y_rp[i] > y_rp[i + 1]) throw Error(GAPHIP_E_ARG, "Y row pointer not monotone");
+    }
+    for (long long k = 0; k < P.y_nnz; ++k)
+        if (y_ci[k] < 0 || y_ci[k] >= P.n_bus) throw Error(GAPHIP_E_ARG, "Y column out of range");
+    for (int c : y_code) {
+        long long idx = c >> 2;
+        if (((c & 3) == kContribShunt && idx >= P.n_app) || ((c & 3) != kContribShunt && idx >= P.n_br))
+            throw Error(GAPHIP_E_ARG, "Y contribution out of range");
+    }
+    for (long long t = 0; t < P.n_items; ++t) {
+        if (it_y[t] < 0 || it_y[t] >= P.y_nnz || it_i[t] < 0 || it_i[t] >= P.n_bus || it_j[t] < 0 ||
+            it_j[t] >= P.n_bus)
+            throw Error(GAPHIP_E_ARG, "Jacobian item out of range");
+        for (int q = 0; q < 4; ++q)
+            if (it_pos[4 * t + q] >= P.j_nnz) throw Error(GAPHIP_E_ARG, "Jacobian slot out of range");
+    }
+    P.bus_inst.upload(bus_inst, s);
+    P.ang.upload(ang, s);
+    P.mag.upload(mag, s);
+    P.peq.upload(peq, s);
+    P.qeq.upload(qeq, s);
+    P.flat.upload(flat, s);
+    P.br_r.upload(br_r, s);
+    P.br_x.upload(br_x, s);
+    P.br_g.upload(br_g, s);
+    P.br_b.upload(br_b, s);
+    P.app_g.upload(app_g, s);
+    P.app_b.upload(app_b, s);
+    P.y_rp.upload(y_rp, s);
+    P.y_ci.upload(y_ci, s);
+    P.y_seg.upload(y_seg, s);
+    P.y_code.upload(y_code, s);
+    P.y_slot_inst.upload(y_slot_inst, s);
+    P.it_i.upload(it_i, s);
+    P.it_j.upload(it_j, s);
+    P.it_y.upload(it_y, s);
+    P.it_pos.upload(it_pos, s);
+    HIPCHK(hipStreamSynchronize(s));
+
+    // fronts
+    std::vector<LUPlan const*> plans(n_inst);
+    std::vector<long long> ent_base(n_inst);
+    for (int g = 0; g < n_inst; ++g) {
+        plans[g] = &P.lp[g];
+        ent_base[g] = P.j_off[g];
+    }
+    auto key_row = [&](int g, int key) {
+        int bus = P.node_bus[g][key >> 1];
+        return P.var_off[g] + ((key & 1) ? P.np[g].qeq[bus] : P.np[g].peq[bus]);
+    };
+    auto key_col = [&](int g, int key) {
+        int bus = P.node_bus[g][key >> 1];
+        return P.var_off[g] + ((key & 1) ? P.np[g].mag[bus] : P.np[g].ang[bus]);
+    };
+    pack_fronts(plans, ent_base, key_row, key_col, P.n_var, P.j_nnz, P.fr, s);
+
+    // per-bus / per-instance state and work arrays
+    P.v.alloc(P.n_bus);
+    P.vpol.alloc(P.n_bus);
+    P.s.alloc(P.n_bus);
+    P.sspec.alloc(P.n_bus);
+    P.y_pu.alloc(P.y_nnz);
+    P.zb.alloc(n_inst);
+    P.done.alloc(n_inst);
+    P.iters.alloc(n_inst);
+    P.lu_fail.alloc(n_inst);
+    P.final_mis.alloc(n_inst);
+    P.f.alloc(std::max<long long>(P.n_var, 1));
+    P.dx.alloc(std::max<long long>(P.n_var, 1));
+    if (precision == GAPHIP_FP64) {
+        P.jv64.alloc(std::max<long long>(P.j_nnz, 1));
+        P.front64.alloc(std::max<long long>(P.fr.front_entries, 1));
+        P.vec64.alloc(std::max<long long>(P.fr.vec_entries, 1));
+    } else {
+        P.jv32.alloc(std::max<long long>(P.j_nnz, 1));
+        P.front32.alloc(std::max<long long>(P.fr.front_entries, 1));
+        P.vec32.alloc(std::max<long long>(P.fr.vec_entries, 1));
+    }
+    P.analysis_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// S_spec in per unit (cpu_newton_raphson.cpp:346-377): bus injection if non-zero,
+// plus every in-service SOURCE / LOADGEN appliance on that bus id in appliance
+// order, divided by base_power.
+void upload_sspec(gaphip_plan& P, double base) {
+    std::vector<double2> sp(P.n_bus);
+    std::vector<double> zb(P.n_inst);
+    for (int g = 0; g < P.n_inst; ++g) {
+        auto const& N = P.nets[g];
+        auto const& Y = P.yp[g];
+        int const n = N.n_bus();
+        std::vector<double> pr(n, 0.0), pi(n, 0.0);
+        for (int i = 0; i < n; ++i)
+            if (N.p_w[i] != 0.0 || N.q_var[i] != 0.0) {
+                pr[i] = N.p_w[i];
+                pi[i] = N.q_var[i];
+            }
+        for (int a = 0; a < N.n_app(); ++a) {
+            if (N.app_status[a] != 1 || (N.app_type[a] != kSource && N.app_type[a] != kLoadgen)) continue;
+            int id = N.app_node[a];
+            if (id < 0 || id >= static_cast<int>(Y.id2idx.size())) continue;
+            int i = Y.id2idx[id];
+            if (i < 0) continue;
+            pr[i] += N.app_p[a];
+            pi[i] += N.app_q[a];
+        }
+        for (int i = 0; i < n; ++i) {
+            double r = pr[i], im = pi[i];
+            if (base > 0.0) {
+                r /= base;
+                im /= base;
+            }
+            sp[P.bus_off[g] + i] = make_double2(r, im);
+        }
+        double vb = N.u_rated[0];
+        zb[g] = (vb * vb) / base;
+    }
+    std::vector<double2> const& c = sp;
+    HIPCHK(hipMemcpyAsync(P.sspec.p, c.data(), c.size() * sizeof(double2), hipMemcpyHostToDevice, P.stream));
+    HIPCHK(hipMemcpyAsync(P.zb.p, zb.data(), zb.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
+    HIPCHK(hipStreamSynchronize(P.stream));
+    P.sspec_dirty = false;
+    P.sspec_base = base;
+}
+
+// Overwrite the assembled per-unit Y of instances that were given a Y-bus.
+void apply_given_y(gaphip_plan& P, double base) {
+    if (P.y_given_mode.empty()) return;
+    for (int g = 0; g < P.n_inst; ++g) {
+        int mode = P.y_given_mode[g];
+        if (!mode) continue;
+        double vb = P.nets[g].u_rated[0];
+        double scale = mode == 1 ? (vb * vb) / base : 1.0;
+        int b = P.y_off[g], e = P.y_off[g + 1];
+        if (e > b)
+            hipLaunchKernelGGL(k_yscale, dim3(blocks_for(e - b, 256)), dim3(256), 0, P.stream, b, e, P.y_given.p,
+                               scale, P.y_pu.p);
+    }
+}
+
+// Launch helper that optionally brackets a launch with timing events.
+template <typename F>
+void launch(gaphip_plan& P, int kid, double bytes, F&& fn) {
+    if (P.prof.on) {
+        Profiler::Rec r{kid, P.prof.get(), P.prof.get()};
+        HIPCHK(hipEventRecord(r.a, P.stream));
+        fn();
+        HIPCHK(hipEventRecord(r.b, P.stream));
+        P.prof.recs.push_back(r);
+        P.prof.bytes[kid] += bytes;
+    } else {
+        fn();
+    }
+    HIPCHK(hipGetLastError());
+}
+
+template <typename T>
+void run_solve(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front, T* vec) {
+    hipStream_t s = P.stream;
+    int const G = P.n_inst;
+    int const max_it = o.max_iterations;
+    if (P.sspec_dirty || P.sspec_base != o.base_power) upload_sspec(P, o.base_power);
+    int rows = max_it + 1;
+    if (P.maxf_rows < rows) {
+        P.maxf.alloc(static_cast<size_t>(rows) * G);
+        P.active.alloc(rows);
+        P.maxf_rows = rows;
+    }
+    HIPCHK(hipMemsetAsync(P.maxf.p, 0, sizeof(unsigned long long) * rows * G, s));
+    HIPCHK(hipMemsetAsync(P.active.p, 0, sizeof(int) * rows, s));
+    HIPCHK(hipMemsetAsync(P.done.p, 0, sizeof(int) * G, s));
+    HIPCHK(hipMemsetAsync(P.iters.p, 0, sizeof(int) * G, s));
+    HIPCHK(hipMemsetAsync(P.lu_fail.p, 0, sizeof(int) * G, s));
+    HIPCHK(hipMemsetAsync(P.final_mis.p, 0, sizeof(double) * G, s));
+    if (P.prof.on) P.prof.reset();
+    double const w = sizeof(T);
+    long long const N = P.n_bus, Z = P.y_nnz, V = P.n_var, J = P.j_nnz;
+
+    YbusArgs ya{static_cast<int>(Z), P.y_seg.p, P.y_code.p, P.y_slot_inst.p, P.br_r.p, P.br_x.p, P.br_g.p,
+                P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.y_pu.p, nullptr};
+    launch(P, K_YBUS, 48.0 * Z, [&] { hipLaunchKernelGGL(k_ybus, dim3(blocks_for(Z, 256)), dim3(256), 0, s, ya); });
+    apply_given_y(P, o.base_power);
+    FlatArgs fa{static_cast<int>(N), P.flat.p, P.v.p};
+    launch(P, K_FLAT, 24.0 * N, [&] { hipLaunchKernelGGL(k_flat, dim3(blocks_for(N, 256)), dim3(256), 0, s, fa); });
+
+    FrontArgs fra = P.fr.args(P.done.p, P.lu_fail.p);
+    int it = 0;
+    for (; it < max_it; ++it) {
+        MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
+                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p + static_cast<size_t>(it) * G};
+        launch(P, K_MISMATCH, 20.0 * Z + 56.0 * N + 8.0 * V,
+               [&] { hipLaunchKernelGGL(k_mismatch, dim3(blocks_for(N, 256)), dim3(256), 0, s, ma); });
+        CheckArgs ca{G, it, o.tolerance, P.maxf.p + static_cast<size_t>(it) * G, P.done.p, P.iters.p,
+                     P.lu_fail.p, P.final_mis.p, P.active.p + it};
+        launch(P, K_CHECK, 24.0 * G, [&] { hipLaunchKernelGGL(k_check, dim3(blocks_for(G, 256)), dim3(256), 0, s, ca); });
+        HIPCHK(hipMemcpyAsync(P.pinned_active, P.active.p + it, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(P.ev, s));
+        HIPCHK(hipEventSynchronize(P.ev));
+        if (*P.pinned_active == 0) break;
+        JacArgs ja{static_cast<int>(P.n_items), P.it_i.p, P.it_j.p, P.it_y.p, P.it_pos.p, P.bus_inst.p, P.done.p,
+                   P.y_pu.p, P.s.p, P.vpol.p};
+        launch(P, K_JAC, 20.0 * Z + 44.0 * N + w * J, [&] {
+            hipLaunchKernelGGL(k_jacobian<T>, dim3(blocks_for(P.n_items, 256)), dim3(256), 0, s, ja, jv);
+        });
+        auto hook = [&](int kid, double bytes, auto&& fn) { launch(P, kid, bytes, fn); };
+        FrontRunner<T>::factor(P.fr, fra, jv, front, s, K_FACTOR, hook);
+        FrontRunner<T>::forward(P.fr, fra, front, vec, P.f.p, -1.0, s, K_FWD, hook);
+        FrontRunner<T>::backward(P.fr, fra, front, vec, P.dx.p, s, K_BWD, hook);
+        UpdArgs ua{static_cast<int>(N), P.bus_inst.p, P.done.p, P.lu_fail.p, P.ang.p, P.mag.p, P.dx.p, P.v.p};
+        launch(P, K_UPDATE, 8.0 * V + 32.0 * N,
+               [&] { hipLaunchKernelGGL(k_update, dim3(blocks_for(N, 256)), dim3(256), 0, s, ua); });
+    }
+    P.last_iters_run = std::min(it + 1, max_it);
+    HIPCHK(hipStreamSynchronize(s));
+    if (P.prof.on) P.prof.collect();
+    P.have_solution = true;
+    P.last_base = o.base_power;
+}
+
+}  // namespace gaphip
+
+using namespace gaphip;
+
+#define GAPHIP_GUARD(...)                                     \
+    try {                                                     \
+        __VA_ARGS__;                                          \
+        return GAPHIP_OK;                                     \
+    } catch (gaphip::Error const& e) {                        \
+        gaphip::g_last_error = e.what();                      \
+        return e.code;                                        \
+    } catch (std::bad_alloc const&) {                         \
+        gaphip::g_last_error = "host out of memory";          \
+        return GAPHIP_E_DEVICE;                               \
+    } catch (std::exception const& e) {                       \
+        gaphip::g_last_error = e.what();                      \
+        return GAPHIP_E_ARG;                                  \
+    }
+
+extern "C" {
+
+int gaphip_abi_version(void) { return GAPHIP_ABI_VERSION; }
+const char* gaphip_last_error(void) { return g_last_error.c_str(); }
+
+int gaphip_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return std::strstr(prop.gcnArchName, "gfx950") != nullptr ? 1 : 0;
+}
+
+int gaphip_plan_create(int n_inst, const gaphip_net* nets, int precision, gaphip_plan** out) {
+    GAPHIP_GUARD({
+        if (!out) throw Error(GAPHIP_E_ARG, "out is null");
+        *out = nullptr;
+        auto p = std::make_unique<gaphip_plan>();
+        build_plan(*p, n_inst, nets, precision);
+        *out = p.release();
+    })
+}
+
+int gaphip_analyze(int n_inst, const gaphip_net* nets, gaphip_plan_stats* o) {
+    GAPHIP_GUARD({
+        if (!o) throw Error(GAPHIP_E_ARG, "out is null");
+        gaphip_plan P;
+        analyze(P, n_inst, nets, GAPHIP_FP64);
+        *o = gaphip_plan_stats{};
+        o->n_inst = P.n_inst;
+        o->n_bus = P.n_bus;
+        o->n_branch = P.n_br;
+        o->ybus_nnz = P.y_nnz;
+        o->n_var = P.n_var;
+        o->jac_nnz = P.j_nnz;
+        for (auto const& L : P.lp) {
+            o->n_fronts += L.n_fronts;
+            o->n_levels = std::max<long long>(o->n_levels, L.n_levels);
+            o->front_entries += L.front_entries;
+            o->factor_entries += L.factor_entries;
+            o->max_front = std::max<long long>(o->max_front, L.max_front);
+        }
+        o->analysis_ms = P.analysis_ms;
+    })
+}
+
+int gaphip_plan_destroy(gaphip_plan* plan) {
+    GAPHIP_GUARD({ delete plan; })
+}
+
+int gaphip_plan_stats_get(const gaphip_plan* P, gaphip_plan_stats* o) {
+    GAPHIP_GUARD({
+        if (!P || !o) throw Error(GAPHIP_E_ARG, "null argument");
+        o->n_inst = P->n_inst;
+        o->n_bus = P->n_bus;
+        o->n_branch = P->n_br;
+        o->ybus_nnz = P->y_nnz;
+        o->n_var = P->n_var;
+        o->jac_nnz = P->j_nnz;
+        o->n_fronts = P->fr.n_fronts;
+        o->n_levels = P->fr.n_levels;
+        o->front_entries = P->fr.front_entries;
+        o->factor_entries = P->fr.factor_entries;
+        o->max_front = P->fr.max_front;
+        o->analysis_ms = P->analysis_ms;
+    })
+}
+
+int gaphip_plan_set_injections(gaphip_plan* P, int inst, const double* p_w, const double* q_var) {
+    GAPHIP_GUARD({
+        if (!P || inst < 0 || inst >= P->n_inst || !p_w || !q_var) throw Error(GAPHIP_E_ARG, "bad argument");
+        auto& N = P->nets[inst];
+        N.p_w.assign(p_w, p_w + N.n_bus());
+        N.q_var.assign(q_var, q_var + N.n_bus());
+        P->sspec_dirty = true;
+    })
+}
+
+int gaphip_plan_set_branch_rx(gaphip_plan* P, int inst, const double* r, const double* x) {
+    GAPHIP_GUARD({
+        if (!P || inst < 0 || inst >= P->n_inst || !r || !x) throw Error(GAPHIP_E_ARG, "bad argument");
+        auto& N = P->nets[inst];
+        int const m = N.n_branch();
+        N.br_r.assign(r, r + m);
+        N.br_x.assign(x, x + m);
+        if (m) {
+            HIPCHK(hipMemcpyAsync(P->br_r.p + P->br_off[inst], r, sizeof(double) * m, hipMemcpyHostToDevice,
+                                  P->stream));
+            HIPCHK(hipMemcpyAsync(P->br_x.p + P->br_off[inst], x, sizeof(double) * m, hipMemcpyHostToDevice,
+                                  P->stream));
+            HIPCHK(hipStreamSynchronize(P->stream));
+        }
+    })
+}
+
+int gaphip_plan_set_ybus(gaphip_plan* P, int inst, const int* rp, const int* ci, const double* v_ri, int nnz,
+                         int per_unit) {
+    GAPHIP_GUARD({
+        if (!P || inst < 0 || inst >= P->n_inst) throw Error(GAPHIP_E_ARG, "bad argument");
+        if (P->y_given_mode.empty()) P->y_given_mode.assign(P->n_inst, 0);
+        if (!rp) {
+            P->y_given_mode[inst] = 0;
+            return GAPHIP_OK;
+        }
+        auto const& Y = P->yp[inst];
+        int const n = Y.n;
+        if (!ci || !v_ri || nnz != static_cast<int>(Y.ci.size()) || !std::equal(rp, rp + n + 1, Y.rp.begin()) ||
+            !std::equal(ci, ci + nnz, Y.ci.begin()))
+            throw Error(GAPHIP_E_ARG, "given Y-bus pattern differs from the network's admittance pattern");
+        if (!P->y_given.p) P->y_given.alloc(std::max<long long>(P->y_nnz, 1));
+        if (nnz)
+            HIPCHK(hipMemcpyAsync(P->y_given.p + P->y_off[inst], v_ri, sizeof(double2) * nnz, hipMemcpyHostToDevice,
+                                  P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        P->y_given_mode[inst] = per_unit ? 2 : 1;
+    })
+}
+
+int gaphip_solve(gaphip_plan* P, const gaphip_solve_opts* o, double* v_ri, int* iterations, int* converged,
+                 double* final_mismatch) {
+    GAPHIP_GUARD({
+        if (!P || !o) throw Error(GAPHIP_E_ARG, "null argument");
+        if (o->max_iterations < 0) throw Error(GAPHIP_E_ARG, "max_iterations must be >= 0");
+        if (!(o->base_power > 0.0)) throw Error(GAPHIP_E_ARG, "base_power must be > 0");
+        if (P->precision == GAPHIP_FP64)
+            run_solve<double>(*P, *o, P->jv64.p, P->front64.p, P->vec64.p);
+        else
+            run_solve<float>(*P, *o, P->jv32.p, P->front32.p, P->vec32.p);
+        int const G = P->n_inst;
+        std::vector<int> done(G), its(G);
+        HIPCHK(hipMemcpy(done.data(), P->done.p, sizeof(int) * G, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(its.data(), P->iters.p, sizeof(int) * G, hipMemcpyDeviceToHost));
+        for (int g = 0; g < G; ++g)
+            if (!done[g]) its[g] = o->max_iterations;
+        if (iterations) std::copy(its.begin(), its.end(), iterations);
+        if (converged) std::copy(done.begin(), done.end(), converged);
+        if (final_mismatch) HIPCHK(hipMemcpy(final_mismatch, P->final_mis.p, sizeof(double) * G, hipMemcpyDeviceToHost));
+        if (v_ri) HIPCHK(hipMemcpy(v_ri, P->v.p, sizeof(double2) * P->n_bus, hipMemcpyDeviceToHost));
+        P->maxf_host.clear();
+    })
+}
+
+int gaphip_trajectory(const gaphip_plan* P, int inst, double* out, int cap, int* n_out) {
+    GAPHIP_GUARD({
+        if (!P || inst < 0 || inst >= P->n_inst || !n_out) throw Error(GAPHIP_E_ARG, "bad argument");
+        if (!P->have_solution) throw Error(GAPHIP_E_STATE, "no solve has run");
+        int const G = P->n_inst, rows = P->last_iters_run;
+        std::vector<unsigned long long> h(static_cast<size_t>(rows) * G);
+        if (rows) HIPCHK(hipMemcpy(h.data(), P->maxf.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+        std::vector<int> its(G), done(G);
+        HIPCHK(hipMemcpy(its.data(), P->iters.p, sizeof(int) * G, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(done.data(), P->done.p, sizeof(int) * G, hipMemcpyDeviceToHost));
+        int n = done[inst] ? its[inst] : rows;
+        *n_out = n;
+        for (int k = 0; k < n && k < cap; ++k) {
+            long long b = static_cast<long long>(h[static_cast<size_t>(k) * G + inst]);
+            double d;
+            std::memcpy(&d, &b, sizeof d);
+            out[k] = d;
+        }
+    })
+}
+
+int gaphip_profile_enable(gaphip_plan* P, int on) {
+    GAPHIP_GUARD({
+        if (!P) throw Error(GAPHIP_E_ARG, "null plan");
+        P->prof.on = on != 0;
+        P->prof.reset();
+    })
+}
+
+int gaphip_profile_get(const gaphip_plan* P, int k, double* ms, long long* launches, double* bytes) {
+    GAPHIP_GUARD({
+        if (!P || k < 0 || k >= K_COUNT) throw Error(GAPHIP_E_ARG, "bad kernel id");
+        if (ms) *ms = P->prof.ms[k];
+        if (launches) *launches = P->prof.launches[k];
+        if (bytes) *bytes = P->prof.bytes[k];
+    })
+}
+
+const char* gaphip_kernel_name(int k) { return (k >= 0 && k < K_COUNT) ? kKernelNames[k] : ""; }
+int gaphip_kernel_count(void) { return K_COUNT; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------ inspection
+namespace gaphip {
+
+// Evaluates Y, S, F and (optionally) J of one instance at the given voltages.
+void evaluate_instance(gaphip_plan& P, int inst, double const* v_ri, double base, bool want_j) {
+    hipStream_t s = P.stream;
+    if (P.sspec_dirty || P.sspec_base != base) upload_sspec(P, base);
+    if (P.maxf_rows < 1) {
+        P.maxf.alloc(P.n_inst);
+        P.active.alloc(1);
+        P.maxf_rows = 1;
+    }
+    int const G = P.n_inst;
+    std::vector<int> done(G, 1);
+    done[inst] = 0;
+    HIPCHK(hipMemcpyAsync(P.done.p, done.data(), sizeof(int) * G, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(P.maxf.p, 0, sizeof(unsigned long long) * G, s));
+    int const nb = P.nets[inst].n_bus();
+    HIPCHK(hipMemcpyAsync(P.v.p + P.bus_off[inst], v_ri, sizeof(double2) * nb, hipMemcpyHostToDevice, s));
+    YbusArgs ya{static_cast<int>(P.y_nnz), P.y_seg.p, P.y_code.p, P.y_slot_inst.p, P.br_r.p, P.br_x.p, P.br_g.p,
+                P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.y_pu.p, nullptr};
+    hipLaunchKernelGGL(k_ybus, dim3(blocks_for(P.y_nnz, 256)), dim3(256), 0, s, ya);
+    apply_given_y(P, base);
+    MisArgs ma{static_cast<int>(P.n_bus), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
+               P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p};
+    hipLaunchKernelGGL(k_mismatch, dim3(blocks_for(P.n_bus, 256)), dim3(256), 0, s, ma);
+    if (want_j) {
+        if (!P.jv64.p) P.jv64.alloc(std::max<long long>(P.j_nnz, 1));
+        JacArgs ja{static_cast<int>(P.n_items), P.it_i.p, P.it_j.p, P.it_y.p, P.it_pos.p, P.bus_inst.p, P.done.p,
+                   P.y_pu.p, P.s.p, P.vpol.p};
+        hipLaunchKernelGGL(k_jacobian<double>, dim3(blocks_for(P.n_items, 256)), dim3(256), 0, s, ja, P.jv64.p);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    P.have_solution = false;
+}
+
+}  // namespace gaphip
+
+extern "C" {
+
+int gaphip_jacobian(gaphip_plan* P, int inst, const double* v_ri, double base_power, int* n_var, int* row_ptr,
+                    int* col_idx, double* values, int cap, int* nnz_out) {
+    GAPHIP_GUARD({
+        if (!P || inst < 0 || inst >= P->n_inst || !v_ri || !n_var || !nnz_out)
+            throw Error(GAPHIP_E_ARG, "bad argument");
+        if (!(base_power > 0.0)) throw Error(GAPHIP_E_ARG, "base_power must be > 0");
+        evaluate_instance(*P, inst, v_ri, base_power, true);
+        auto const& R = P->np[inst];
+        int const nz = static_cast<int>(R.jci.size());
+        std::vector<double> jv(nz);
+        if (nz)
+            HIPCHK(hipMemcpy(jv.data(), P->jv64.p + P->j_off[inst], sizeof(double) * nz, hipMemcpyDeviceToHost));
+        // reference compaction: keep |J| > 1e-12 (cpu_newton_raphson.cpp:752-761)
+        int k = 0;
+        *n_var = R.n_var;
+        if (row_ptr) row_ptr[0] = 0;
+        for (int r = 0; r < R.n_var; ++r) {
+            for (int q = R.jrp[r]; q < R.jrp[r + 1]; ++q) {
+                if (!(std::fabs(jv[q]) > 1e-12)) continue;
+                if (k < cap && col_idx && values) {
+                    col_idx[k] = R.jci[q];
+                    values[k] = jv[q];
+                }
+                ++k;
+            }
+            if (row_ptr) row_ptr[r + 1] = k;
+        }
+        *nnz_out = k;
+        if (k > cap) throw Error(GAPHIP_E_CAPACITY, "Jacobian capacity too small");
+    })
+}
+
+int gaphip_mismatches(gaphip_plan* P, int inst, const double* v_ri, double base_power, double* f, int cap,
+                      int* m_out) {
+    GAPHIP_GUARD({
+        if (!P || inst < 0 || inst >= P->n_inst || !v_ri || !m_out) throw Error(GAPHIP_E_ARG, "bad argument");
+        if (!(base_power > 0.0)) throw Error(GAPHIP_E_ARG, "base_power must be > 0");
+        evaluate_instance(*P, inst, v_ri, base_power, false);
+        int const m = P->np[inst].n_var;
+        *m_out = m;
+        if (m > cap) throw Error(GAPHIP_E_CAPACITY, "mismatch capacity too small");
+        if (m) HIPCHK(hipMemcpy(f, P->f.p + P->var_off[inst], sizeof(double) * m, hipMemcpyDeviceToHost));
+    })
+}
+
+int gaphip_ybus_build(const gaphip_net* net, int* row_ptr, int* col_idx, double* values_ri, int cap, int* nnz_out) {
+    GAPHIP_GUARD({
+        if (!net || !row_ptr || !nnz_out) throw Error(GAPHIP_E_ARG, "null argument");
+        int dev_count = 0;
+        if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+            throw Error(GAPHIP_E_DEVICE, "no HIP device visible");
+        Net N = copy_net(*net);
+        YPlan Y = plan_ybus(N);
+        int const nnz = static_cast<int>(Y.ci.size());
+        *nnz_out = nnz;
+        std::copy(Y.rp.begin(), Y.rp.end(), row_ptr);
+        if (nnz > cap) throw Error(GAPHIP_E_CAPACITY, "Y-bus capacity too small");
+        std::copy(Y.ci.begin(), Y.ci.end(), col_idx);
+        hipStream_t s;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } guard{s};
+        DBuf<int> seg, code, slot_inst;
+        DBuf<double> r, x, g, b, ag, ab, zb;
+        DBuf<double2> ypu, ysi;
+        seg.upload(Y.seg, s);
+        code.upload(Y.code, s);
+        slot_inst.upload(std::vector<int>(nnz, 0), s);
+        r.upload(N.br_r, s);
+        x.upload(N.br_x, s);
+        g.upload(N.br_g, s);
+        b.upload(N.br_b, s);
+        ag.upload(N.app_g.empty() ? std::vector<double>{0.0} : N.app_g, s);
+        ab.upload(N.app_b.empty() ? std::vector<double>{0.0} : N.app_b, s);
+        zb.upload(std::vector<double>{1.0}, s);
+        ypu.alloc(std::max(nnz, 1));
+        ysi.alloc(std::max(nnz, 1));
+        YbusArgs ya{nnz, seg.p, code.p, slot_inst.p, r.p, x.p, g.p, b.p, ag.p, ab.p, zb.p, ypu.p, ysi.p};
+        if (nnz) hipLaunchKernelGGL(k_ybus, dim3(blocks_for(nnz, 256)), dim3(256), 0, s, ya);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        if (nnz) HIPCHK(hipMemcpy(values_ri, ysi.p, sizeof(double2) * nnz, hipMemcpyDeviceToHost));
+    })
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ general LU
+struct gaphip_lu {
+    int n = 0;
+    bool factorized = false;
+    std::vector<int> pat_rp, pat_ci;  // pattern the symbolic plan was built for
+    std::vector<int> uniq_of;         // CSR entry -> unique (i, j) index
+    int n_uniq = 0;
+    gaphip::LUPlan plan;
+    gaphip::DevFronts fr;
+    gaphip::DBuf<double> vals, front, vec, rhs, sol;
+    gaphip::DBuf<int> fail;
+    hipStream_t stream = nullptr;
+    ~gaphip_lu() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace gaphip {
+
+// Complex matrix as a real system of twice the size: node i carries (re, im),
+// entry a+ib of (i, j) becomes [[a, -b], [b, a]] on unknowns (2i, 2i+1) x (2j, 2j+1).
+void lu_symbolic(gaphip_lu& L, int n, int const* rp, int const* ci) {
+    LUInput in;
+    in.n_nodes = n;
+    in.w.assign(n, 2);
+    std::vector<std::vector<int>> nb(n);
+    L.uniq_of.assign(rp[n], -1);
+    std::vector<int> last(n, -1);
+    std::vector<int> urow, ucol;
+    // unique (i, j) in CSR order; duplicates fold onto the first occurrence
+    std::vector<int> first_at(n, -1);
+    for (int i = 0; i < n; ++i) {
+        for (int q = rp[i]; q < rp[i + 1]; ++q) {
+            int j = ci[q];
+            if (last[j] == i) {
+                L.uniq_of[q] = first_at[j];
+                continue;
+            }
+            last[j] = i;
+            first_at[j] = static_cast<int>(urow.size());
+            L.uniq_of[q] = first_at[j];
+            urow.push_back(i);
+            ucol.push_back(j);
+            if (i != j) {
+                nb[i].push_back(j);
+                nb[j].push_back(i);
+            }
+        }
+    }
+    L.n_uniq = static_cast<int>(urow.size());
+    in.adj_ptr.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
+        std::sort(nb[i].begin(), nb[i].end());
+        nb[i].erase(std::unique(nb[i].begin(), nb[i].end()), nb[i].end());
+        in.adj_ptr[i + 1] = in.adj_ptr[i] + static_cast<int>(nb[i].size());
+        in.adj.insert(in.adj.end(), nb[i].begin(), nb[i].end());
+    }
+    for (int u = 0; u < L.n_uniq; ++u)
+        for (int t = 0; t < 4; ++t) {
+            in.ent_row.push_back(2 * urow[u] + (t >> 1));
+            in.ent_col.push_back(2 * ucol[u] + (t & 1));
+        }
+    L.plan = plan_lu(in);
+    std::vector<LUPlan const*> plans{&L.plan};
+    std::vector<long long> base{0};
+    auto ident = [](int, int key) { return key; };
+    pack_fronts(plans, base, ident, ident, 2LL * n, 4LL * L.n_uniq, L.fr, L.stream);
+    L.front.alloc(std::max<long long>(L.fr.front_entries, 1));
+    L.vec.alloc(std::max<long long>(L.fr.vec_entries, 1));
+    L.rhs.alloc(2 * n);
+    L.sol.alloc(2 * n);
+    L.fail.alloc(1);
+    L.pat_rp.assign(rp, rp + n + 1);
+    L.pat_ci.assign(ci, ci + rp[n]);
+    L.n = n;
+}
+
+}  // namespace gaphip
+
+extern "C" {
+
+int gaphip_lu_create(gaphip_lu** out) {
+    GAPHIP_GUARD({
+        if (!out) throw Error(GAPHIP_E_ARG, "out is null");
+        auto L = std::make_unique<gaphip_lu>();
+        int dev_count = 0;
+        if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+            throw Error(GAPHIP_E_DEVICE, "no HIP device visible");
+        HIPCHK(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+        *out = L.release();
+    })
+}
+
+int gaphip_lu_destroy(gaphip_lu* L) {
+    GAPHIP_GUARD({ delete L; })
+}
+
+int gaphip_lu_is_factorized(const gaphip_lu* L) { return (L && L->factorized) ? 1 : 0; }
+
+int gaphip_lu_factorize(gaphip_lu* L, int n_rows, int n_cols, const int* rp, const int* ci, const double* v_ri,
+                        int nnz) {
+    GAPHIP_GUARD({
+        if (!L) throw Error(GAPHIP_E_ARG, "null LU handle");
+        L->factorized = false;
+        if (n_rows != n_cols) throw Error(GAPHIP_E_ARG, "Matrix must be square for LU factorization");
+        if (nnz <= 0 || !v_ri || !rp || !ci || n_rows <= 0)
+            throw Error(GAPHIP_E_ARG, "Matrix has no non-zero elements");
+        int const n = n_rows;
+        if (rp[0] != 0 || rp[n] != nnz) throw Error(GAPHIP_E_ARG, "row_ptr inconsistent with nnz");
+        for (int i = 0; i < n; ++i)
+            if (rp[i + 1] < rp[i]) throw Error(GAPHIP_E_ARG, "row_ptr not monotone");
+        for (int q = 0; q < nnz; ++q)
+            if (ci[q] < 0 || ci[q] >= n) throw Error(GAPHIP_E_ARG, "column index out of range");
+        bool same = L->n == n && static_cast<int>(L->pat_ci.size()) == nnz &&
+                    std::equal(rp, rp + n + 1, L->pat_rp.begin()) && std::equal(ci, ci + nnz, L->pat_ci.begin());
+        if (!same) lu_symbolic(*L, n, rp, ci);
+        // expanded real values in unique-entry order, duplicates summed
+        std::vector<double> ur(L->n_uniq, 0.0), ui(L->n_uniq, 0.0);
+        for (int q = 0; q < nnz; ++q) {
+            ur[L->uniq_of[q]] += v_ri[2 * q];
+            ui[L->uniq_of[q]] += v_ri[2 * q + 1];
+        }
+        std::vector<double> ex(4 * static_cast<size_t>(L->n_uniq));
+        for (int u = 0; u < L->n_uniq; ++u) {
+            ex[4 * u + 0] = ur[u];
+            ex[4 * u + 1] = -ui[u];
+            ex[4 * u + 2] = ui[u];
+            ex[4 * u + 3] = ur[u];
+        }
+        L->vals.upload(ex, L->stream);
+        HIPCHK(hipMemsetAsync(L->fail.p, 0, sizeof(int), L->stream));
+        FrontArgs fa = L->fr.args(nullptr, L->fail.p);
+        FrontRunner<double>::factor(L->fr, fa, L->vals.p, L->front.p, L->stream, 0, PlainHook{});
+        HIPCHK(hipGetLastError());
+        int failed = 0;
+        HIPCHK(hipMemcpyAsync(&failed, L->fail.p, sizeof(int), hipMemcpyDeviceToHost, L->stream));
+        HIPCHK(hipStreamSynchronize(L->stream));
+        if (failed) throw Error(GAPHIP_E_SINGULAR, "numerically zero pivot (|pivot| < 1e-14)");
+        L->factorized = true;
+    })
+}
+
+int gaphip_lu_solve(gaphip_lu* L, const double* rhs_ri, double* x_ri, int n) {
+    GAPHIP_GUARD({
+        if (!L) throw Error(GAPHIP_E_ARG, "null LU handle");
+        if (!L->factorized) throw Error(GAPHIP_E_STATE, "Matrix not factorized. Call factorize() first.");
+        if (n != L->n) throw Error(GAPHIP_E_ARG, "RHS size does not match matrix dimension");
+        if (!rhs_ri || !x_ri) throw Error(GAPHIP_E_ARG, "null vector");
+        HIPCHK(hipMemcpyAsync(L->rhs.p, rhs_ri, sizeof(double) * 2 * n, hipMemcpyHostToDevice, L->stream));
+        FrontArgs fa = L->fr.args(nullptr, L->fail.p);
+        FrontRunner<double>::forward(L->fr, fa, L->front.p, L->vec.p, L->rhs.p, 1.0, L->stream, 0, PlainHook{});
+        FrontRunner<double>::backward(L->fr, fa, L->front.p, L->vec.p, L->sol.p, L->stream, 0, PlainHook{});
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(x_ri, L->sol.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, L->stream));
+        HIPCHK(hipStreamSynchronize(L->stream));
+    })
+}
+
+}  // extern "C"
+
+extern "C" int gaphip_adjoint(gaphip_plan* P, double, double, const double*, const double*, double*, double*,
+                              double*, double*, double*) {
+    GAPHIP_GUARD({
+        if (!P) throw Error(GAPHIP_E_ARG, "null plan");
+        throw Error(GAPHIP_E_STATE, "adjoint is not available in this build");
+    })
+}

@@ -1,0 +1,195 @@
+/*
+ * gap_hip.h — C ABI of the MI355X Newton-Raphson power-flow backend.
+ *
+ * Plain C: opaque handles, pointers and sizes only.  The C++ plugin classes
+ * (IAdmittanceMatrix / ILUSolver / IPowerFlowSolver, exported through the
+ * create_gpu_* factories of gap_hip_plugins.h) and the Python module are
+ * written on top of these functions; the HIP kernels sit underneath.
+ *
+ * Every function returns 0 on success and a negative GAPHIP_E* code on
+ * failure; gaphip_last_error() then holds a message (thread-local).
+ *
+ * Units and conventions follow the reference data model
+ * (include/gap/core/types.h): volts, ohms, siemens, watts, VAr, powers in
+ * generator sign; complex numbers are interleaved (re, im) doubles.
+ */
+#ifndef GAP_HIP_H
+#define GAP_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GAPHIP_ABI_VERSION 1
+
+enum {
+    GAPHIP_OK = 0,
+    GAPHIP_E_ARG = -1,      /* invalid argument / inconsistent sizes            */
+    GAPHIP_E_DEVICE = -2,   /* no usable GPU or a HIP runtime error             */
+    GAPHIP_E_CAPACITY = -3, /* caller-provided output buffer too small          */
+    GAPHIP_E_SINGULAR = -4, /* numerically zero pivot during factorisation      */
+    GAPHIP_E_STATE = -5     /* call out of order (e.g. solve before factorize)  */
+};
+
+enum { GAPHIP_PQ = 0, GAPHIP_PV = 1, GAPHIP_SLACK = 2 };           /* types.h:30-34 */
+enum { GAPHIP_SOURCE = 0, GAPHIP_LOADGEN = 1, GAPHIP_SHUNT = 2 };  /* types.h:39-43 */
+enum { GAPHIP_FP64 = 0, GAPHIP_MIXED = 1 }; /* factor/solve precision: FP64, or FP32 with FP64 residuals */
+
+/* One network.  Bus, branch and appliance fields mirror BusData (types.h:204-219),
+ * BranchData (types.h:224-274) and ApplianceData (types.h:284-310); bus and
+ * appliance endpoints are ids, resolved as in cpu_admittance_matrix.cpp:56-73.
+ * br_g / br_b may be NULL (zero). */
+typedef struct gaphip_net {
+    int n_bus;
+    const int* bus_id;
+    const double* u_rated;
+    const int* bus_type;
+    const double* u_pu;
+    const double* p_w;
+    const double* q_var;
+
+    int n_branch;
+    const int* br_id;
+    const int* br_from;
+    const int* br_to;
+    const int* br_status;
+    const double* br_r;
+    const double* br_x;
+    const double* br_g;
+    const double* br_b;
+
+    int n_app;
+    const int* app_id;
+    const int* app_node;
+    const int* app_status;
+    const int* app_type;
+    const double* app_p;
+    const double* app_q;
+    const double* app_g;
+    const double* app_b;
+} gaphip_net;
+
+int gaphip_abi_version(void);
+const char* gaphip_last_error(void);
+
+/* 1 when a gfx950 device is visible.  Replaces the CUDA availability probe
+ * BackendFactory::check_cuda_availability (src/core/backend_factory.cpp:134-160). */
+int gaphip_device_available(void);
+
+/* ---------------------------------------------------------------- Y-bus
+ * Admittance matrix in siemens, CSR with int32 indices, assembled on the GPU.
+ * Replaces IAdmittanceMatrix::build_admittance_matrix
+ * (include/gap/admittance/admittance_interface.h:21-22; semantics of
+ * src/admittance/cpu/cpu_admittance_matrix.cpp:44-202, bit-identical values).
+ * row_ptr: n_bus+1; col_idx / values_ri: capacity `cap` entries (values 2*cap doubles). */
+int gaphip_ybus_build(const gaphip_net* net, int* row_ptr, int* col_idx, double* values_ri, int cap,
+                      int* nnz_out);
+
+/* ------------------------------------------------------ power-flow plan
+ * A plan holds the symbolic work for a batch of independent networks
+ * (Y-bus assembly map, Jacobian pattern, fill-reducing order, supernodal
+ * fronts and level schedule) plus their device-resident data.  It is the
+ * analysis step that IPowerFlowSolver::solve_power_flow
+ * (include/gap/solver/powerflow_interface.h:59-61) and ILUSolver::factorize
+ * (include/gap/solver/lu_solver_interface.h:21-26) redo on every call in the
+ * reference; here it runs once per topology. */
+typedef struct gaphip_plan gaphip_plan;
+
+typedef struct gaphip_plan_stats {
+    int n_inst;
+    long long n_bus, n_branch, ybus_nnz, n_var, jac_nnz;
+    long long n_fronts, n_levels, front_entries, factor_entries, max_front;
+    double analysis_ms;
+} gaphip_plan_stats;
+
+int gaphip_plan_create(int n_inst, const gaphip_net* nets, int precision, gaphip_plan** out);
+/* Host-only symbolic analysis of a batch (no device touched): fills the same
+ * statistics as gaphip_plan_stats_get so fill, front memory and the level
+ * count can be sized before anything is allocated. */
+int gaphip_analyze(int n_inst, const gaphip_net* nets, gaphip_plan_stats* out);
+int gaphip_plan_destroy(gaphip_plan* plan);
+int gaphip_plan_stats_get(const gaphip_plan* plan, gaphip_plan_stats* out);
+
+/* Replace the specified bus injections (BusData active/reactive power, W / VAr)
+ * of instance `inst` before the next solve; pointers of length n_bus of that instance. */
+int gaphip_plan_set_injections(gaphip_plan* plan, int inst, const double* p_w, const double* q_var);
+/* Replace branch r1 / x1 (ohm) of instance `inst` (length n_branch). */
+int gaphip_plan_set_branch_rx(gaphip_plan* plan, int inst, const double* r, const double* x);
+/* Use a caller-supplied Y-bus for instance `inst` instead of assembling it
+ * (IPowerFlowSolver::solve_power_flow receives Y from the caller,
+ * powerflow_interface.h:59-61).  The pattern must equal the plan's Y-bus
+ * pattern.  per_unit = 0: siemens, scaled by (u_rated[0]^2)/base_power at use
+ * (cpu_newton_raphson.cpp:46-58); per_unit = 1: used as given.
+ * Passing row_ptr = NULL returns the instance to on-device assembly. */
+int gaphip_plan_set_ybus(gaphip_plan* plan, int inst, const int* row_ptr, const int* col_idx,
+                         const double* values_ri, int nnz, int per_unit);
+
+typedef struct gaphip_solve_opts {
+    double tolerance;   /* PowerFlowConfig::tolerance (powerflow_interface.h:14)       */
+    int max_iterations; /* PowerFlowConfig::max_iterations (powerflow_interface.h:15)  */
+    double base_power;  /* PowerFlowConfig::base_power, VA (powerflow_interface.h:19)  */
+} gaphip_solve_opts;
+
+/* Newton-Raphson from flat start on every instance of the plan
+ * (cpu_newton_raphson.cpp:33-218: 0.9-damped, clamped polar update;
+ * iterations = iter+1 at convergence, max_iterations otherwise).
+ * Any output pointer may be NULL (results then stay on the device):
+ *   v_ri            2 * sum(n_bus) doubles, instances concatenated
+ *   iterations      n_inst ints;  converged n_inst ints;  final_mismatch n_inst doubles */
+int gaphip_solve(gaphip_plan* plan, const gaphip_solve_opts* opts, double* v_ri, int* iterations,
+                 int* converged, double* final_mismatch);
+
+/* Per-iteration max|F| of instance `inst` from the last solve (state capture,
+ * powerflow_interface.h:28-35).  Returns the count through n_out. */
+int gaphip_trajectory(const gaphip_plan* plan, int inst, double* max_mismatch, int cap, int* n_out);
+
+/* Jacobian of instance `inst` at voltages v_ri (2*n_bus), in the reference
+ * layout (rows: P then Q per non-slack bus; columns: angles then magnitudes;
+ * |J| > 1e-12 kept, cpu_newton_raphson.cpp:509-765), evaluated by the GPU
+ * kernel with the plan's per-unit Y-bus (base_power as given). */
+int gaphip_jacobian(gaphip_plan* plan, int inst, const double* v_ri, double base_power, int* n_var,
+                    int* row_ptr, int* col_idx, double* values, int cap, int* nnz_out);
+
+/* Mismatch vector of instance `inst` at v_ri (per unit of base_power), same
+ * layout as cpu_newton_raphson.cpp:243-292. */
+int gaphip_mismatches(gaphip_plan* plan, int inst, const double* v_ri, double base_power,
+                      double* f, int cap, int* m_out);
+
+/* Adjoint sensitivities at the converged state of the last solve (FP64):
+ * solves J^T lambda = dL/dx with the factors of J at the solution and returns
+ * dL/d(bus p_w, bus q_var) [per W / VAr] and dL/d(branch r1, x1) [per ohm] for
+ *   L = w_loss * (total active branch loss, pu) + w_dev * sum_PQ(|V_i| - 1)^2
+ *       + sum_i g_vm[i] |V_i| + sum_i g_va[i] arg V_i          (g_* may be NULL).
+ * Arrays are concatenated over instances.  loss_out (n_inst) may be NULL. */
+int gaphip_adjoint(gaphip_plan* plan, double w_loss, double w_dev, const double* g_vm,
+                   const double* g_va, double* grad_p, double* grad_q, double* grad_r,
+                   double* grad_x, double* loss_out);
+
+/* Timing hook for measurement: after each solve, the summed device time and
+ * launch count of each kernel family, and the algorithmic HBM bytes they move.
+ * kernel ids: see gaphip_kernel_name(). Enabled by gaphip_profile_enable(plan, 1). */
+int gaphip_profile_enable(gaphip_plan* plan, int on);
+int gaphip_profile_get(const gaphip_plan* plan, int kernel, double* total_ms, long long* launches,
+                       double* bytes);
+const char* gaphip_kernel_name(int kernel);
+int gaphip_kernel_count(void);
+
+/* ------------------------------------------------------- sparse LU
+ * General sparse LU on the GPU for the ILUSolver plugin
+ * (include/gap/solver/lu_solver_interface.h:12-48): complex CSR in,
+ * factorize / solve / refactor, error contract of cpu_lu_solver.cpp:99-108,
+ * 157-164 (non-square or empty -> E_ARG, solve before factorize -> E_STATE).
+ * The matrix is factored without row exchanges on a minimum-degree order
+ * of pattern(A + A^T); a numerically zero pivot returns GAPHIP_E_SINGULAR. */
+typedef struct gaphip_lu gaphip_lu;
+int gaphip_lu_create(gaphip_lu** out);
+int gaphip_lu_destroy(gaphip_lu* lu);
+int gaphip_lu_factorize(gaphip_lu* lu, int n_rows, int n_cols, const int* row_ptr,
+                        const int* col_idx, const double* values_ri, int nnz);
+int gaphip_lu_solve(gaphip_lu* lu, const double* rhs_ri, double* x_ri, int n);
+int gaphip_lu_is_factorized(const gaphip_lu* lu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -420,7 +420,7 @@ static void min_degree(int n, const int* ap, const int* ai, int* order) {
     hent* h = (hent*)xmalloc(sizeof(hent) * hcap);
     int hn = 0;
     for (int i = 0; i < n; ++i) heap_push(h, &hn, (hent){sz[i], i});
-    int k = 0;
+    int k = 0, stamp = -1;
     while (k < n) {
         hent e = heap_pop(h, &hn);
         int v = e.node;
@@ -436,18 +436,19 @@ static void min_degree(int n, const int* ap, const int* ai, int* order) {
             for (int p = 0; p < sz[u]; ++p)
                 if (adj[u][p] != v) adj[u][w++] = adj[u][p];
             sz[u] = w;
-            for (int p = 0; p < sz[u]; ++p) mark[adj[u][p]] = u;
-            mark[u] = u;
+            ++stamp; /* fresh stamp per (v, u) pair: stale marks must not alias */
+            for (int p = 0; p < sz[u]; ++p) mark[adj[u][p]] = stamp;
+            mark[u] = stamp;
             for (int b = 0; b < dv; ++b) {
                 int x = nb[b];
-                if (mark[x] == u) continue;
+                if (mark[x] == stamp) continue;
                 if (sz[u] == cp[u]) {
                     cp[u] *= 2;
                     adj[u] = (int*)realloc(adj[u], sizeof(int) * (size_t)cp[u]);
                     if (!adj[u]) abort();
                 }
                 adj[u][sz[u]++] = x;
-                mark[x] = u;
+                mark[x] = stamp;
             }
         }
         for (int a = 0; a < dv; ++a) {
@@ -782,6 +783,46 @@ static void update_voltages(const pf_net* net, double* vr, double* vi, const dou
 
 /* ------------------------------------------------------------------ driver */
 
+static int* eq_to_var = NULL; /* equation row -> variable index, set by permute_rows_to_vars */
+
+/* Row permutation putting P_b at theta_b's index and Q_b at |V|_b's; returns 0
+ * when the counts do not line up (several slack buses), leaving J unpermuted. */
+static int permute_rows_to_vars(const pf_net* net, const rcsr* J, rcsr* P) {
+    int n = net->n_bus, ns = 0;
+    for (int i = 0; i < n; ++i) ns += net->bus_type[i] == PF_SLACK;
+    if (ns != 1) return 0;
+    int nv = J->n;
+    free(eq_to_var);
+    eq_to_var = (int*)xmalloc(sizeof(int) * (size_t)(nv ? nv : 1));
+    int slack = first_slack(net);
+    int a = 0, eq = 0, npq = 0;
+    for (int i = 0; i < n; ++i) npq += net->bus_type[i] == PF_PQ;
+    int nang = n - 1;
+    int mg = nang;
+    for (int i = 0; i < n; ++i) {
+        if (i == slack) continue;
+        eq_to_var[eq++] = a++;
+        if (net->bus_type[i] == PF_PQ) eq_to_var[eq++] = mg++;
+    }
+    if (eq != nv || mg != nang + npq) return 0;
+    P->n = nv;
+    P->nnz = J->nnz;
+    P->cap = J->nnz;
+    P->rp = (int*)xcalloc((size_t)nv + 1, sizeof(int));
+    P->ci = (int*)xmalloc(sizeof(int) * (size_t)(J->nnz ? J->nnz : 1));
+    P->v = (double*)xmalloc(sizeof(double) * (size_t)(J->nnz ? J->nnz : 1));
+    for (int r = 0; r < nv; ++r) P->rp[eq_to_var[r] + 1] = J->rp[r + 1] - J->rp[r];
+    for (int r = 0; r < nv; ++r) P->rp[r + 1] += P->rp[r];
+    for (int r = 0; r < nv; ++r) {
+        int d = P->rp[eq_to_var[r]];
+        for (int p = J->rp[r]; p < J->rp[r + 1]; ++p, ++d) {
+            P->ci[d] = J->ci[p];
+            P->v[d] = J->v[p];
+        }
+    }
+    return 1;
+}
+
 static void scale_y(ccsr* y, double zb) {
     for (int k = 0; k < y->nnz; ++k) {
         y->re[k] = y->re[k] * zb;
@@ -850,12 +891,26 @@ int pfo_solve(const pf_net* net, double tol, int max_it, double base_power, doub
         if (J.nnz > 0 && nv == m) {
             double* ai = (double*)xcalloc((size_t)J.nnz, sizeof(double));
             lufac fac;
-            if (lu_factor(nv, J.rp, J.ci, J.v, ai, &fac) == 0) {
+            /* Row order of J is (P_b, Q_b) per bus while its column order is all
+             * angles then all magnitudes; moving row P_b to the index of column
+             * theta_b and Q_b to |V|_b gives the same solution with a structurally
+             * symmetric matrix, so the symmetric ordering and the diagonal-preferring
+             * pivot keep fill low. */
+            rcsr P;
+            int perm_ok = permute_rows_to_vars(net, &J, &P);
+            const rcsr* A = perm_ok ? &P : &J;
+            if (lu_factor(nv, A->rp, A->ci, A->v, ai, &fac) == 0) {
                 for (int k = 0; k < m; ++k) {
-                    rr[k] = -f[k];
-                    ri[k] = 0.0;
+                    int dst = perm_ok ? eq_to_var[k] : k;
+                    rr[dst] = -f[k];
+                    ri[dst] = 0.0;
                 }
                 lu_solve_fac(&fac, rr, ri, xr, xi);
+            }
+            if (perm_ok) {
+                free(P.rp);
+                free(P.ci);
+                free(P.v);
             }
             lu_free(&fac);
             free(ai);

@@ -1,0 +1,43 @@
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+GOLDEN = ROOT / "tests" / "golden"
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "oracle"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import pyoracle
+    if not pyoracle.ORACLE_SO.exists():
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+    return pyoracle.Oracle()
+
+
+def golden_cases(prefix=""):
+    return sorted(p.stem for p in GOLDEN.glob(f"{prefix}*.npz") if not p.stem.startswith("lu_"))
+
+
+def load_golden(name):
+    from gap_amd.network import Network
+    d = np.load(GOLDEN / f"{name}.npz", allow_pickle=False)
+    net = Network(**{k[4:]: d[k] for k in d.files if k.startswith("net_")})
+    return net, d
+
+
+@pytest.fixture(scope="session")
+def device():
+    from gap_amd import capi
+    if not capi.device_available():
+        raise RuntimeError("GPU test run without a visible gfx950 device")
+    return True
