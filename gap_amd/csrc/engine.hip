@@ -65,13 +65,15 @@ struct DBuf {
 inline int blocks_for(long long n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
 
 // Kernel ids of the timing hook: one per kernel (front kernels per size class).
-enum Kid { K_YBUS = 0, K_FLAT, K_MISMATCH, K_CHECK, K_JAC, K_UPDATE, K_FACTOR, K_FWD = K_FACTOR + 5, K_BWD = K_FWD + 5,
-           K_COUNT = K_BWD + 5 };
+enum Kid { K_YBUS = 0, K_FLAT, K_MISMATCH, K_CHECK, K_JAC, K_UPDATE, K_FACTOR, K_FWD = K_FACTOR + 6, K_BWD = K_FWD + 6,
+           K_COUNT = K_BWD + 6 };
 static char const* const kKernelNames[K_COUNT] = {
     "k_ybus", "k_flat", "k_mismatch", "k_check", "k_jacobian", "k_update",
-    "k_front_factor_lds<16>", "k_front_factor_lds<32>", "k_front_factor_lds<64>", "k_front_factor_lds<128>",
-    "k_big_factor (assemble+panel+trsm+gemm)", "k_front_fwd_lds<16>", "k_front_fwd_lds<32>", "k_front_fwd_lds<64>",
-    "k_front_fwd_lds<128>", "k_front_fwd", "k_front_bwd_lds<16>", "k_front_bwd_lds<32>", "k_front_bwd_lds<64>",
+    "k_front_factor_lds<8>", "k_front_factor_lds<16>", "k_front_factor_lds<32>", "k_front_factor_lds<64>",
+    "k_front_factor_lds<128>", "k_big_factor (assemble+panel+trsm+gemm)",
+    "k_front_fwd_lds<8>", "k_front_fwd_lds<16>", "k_front_fwd_lds<32>", "k_front_fwd_lds<64>",
+    "k_front_fwd_lds<128>", "k_front_fwd",
+    "k_front_bwd_lds<8>", "k_front_bwd_lds<16>", "k_front_bwd_lds<32>", "k_front_bwd_lds<64>",
     "k_front_bwd_lds<128>", "k_front_bwd"};
 
 Net copy_net(gaphip_net const& in) {
@@ -178,246 +180,244 @@ LUInput nr_lu_input(Net const& net, YPlan const& y, NRPlan const& p, std::vector
     return in;
 }
 
-// Device copy of a (possibly batched) multifrontal plan.
-// Front size classes: LDS-resident kernels for fronts of up to 16 / 32 / 64 /
-// 128 slots, the global-memory kernel above that.
-constexpr int kNumClasses = 5;
-constexpr int kClassMax[kNumClasses - 1] = {16, 32, 64, 128};
+// Front size classes: LDS-resident kernels for fronts of up to 8 / 16 / 32 /
+// 64 / 128 slots (8, 4, 2, 1, 1 fronts per workgroup), the blocked
+// global-memory path above that.
+constexpr int kNumClasses = 6;
+constexpr int kClassMax[kNumClasses - 1] = {8, 16, 32, 64, 128};
+constexpr int kClassPerBlock[kNumClasses - 1] = {8, 4, 2, 1, 1};  // fronts per workgroup
 inline int front_class(int m) {
     for (int c = 0; c < kNumClasses - 1; ++c)
         if (m <= kClassMax[c]) return c;
     return kNumClasses - 1;
 }
 
+// Device copy of a (possibly batched) multifrontal plan.  Fronts are numbered
+// in (level, size class) order so a launch covers a contiguous id range.
 struct DevFronts {
     int n_fronts = 0, n_levels = 0;
     std::vector<int> level_ptr;  // host copy of the level schedule
-    // fronts of level l and class c: level_front[cls_ptr[l*K+c] .. cls_ptr[l*K+c+1])
+    // fronts of level l and class c: ids [cls_ptr[l*K+c], cls_ptr[l*K+c+1])
     std::vector<int> cls_ptr;
     std::vector<long long> cls_arrow, cls_factor, cls_slots;  // per (level, class) totals
     std::vector<int> cls_mmax, cls_pmax;                      // per (level, class) largest m / p
-    DBuf<int> bigbase, arow, crow, crow_c, crow_a;            // row maps of fronts above 128 slots
-    DBuf<int> level_front, f_m, f_p, f_slot_ptr, f_child_ptr, child, f_rel_off, rel, f_arrow_ptr, arrow_src,
-        arrow_dst, f_inst, slot_row, slot_col;
-    DBuf<long long> f_off, f_voff;
+    DBuf<FrontMeta> meta;
+    DBuf<ChildInfo> cinfo;
+    DBuf<int> rel, arrow_src, arrow_dst, slot_row, slot_col;
+    DBuf<int> bigbase, arow, crow, crow_ci;  // row maps of fronts above 128 slots
     long long front_entries = 0, vec_entries = 0, factor_entries = 0;
     int max_front = 0;
 
-    BigMaps big() const { return BigMaps{bigbase.p, arow.p, crow.p, crow_c.p, crow_a.p}; }
+    BigMaps big() const { return BigMaps{bigbase.p, arow.p, crow.p, crow_ci.p}; }
 
     FrontArgs args(int const* done, int* fail) const {
         FrontArgs a;
-        a.level_front = level_front.p;
-        a.f_m = f_m.p;
-        a.f_p = f_p.p;
-        a.f_off = f_off.p;
-        a.f_voff = f_voff.p;
-        a.f_slot_ptr = f_slot_ptr.p;
-        a.f_child_ptr = f_child_ptr.p;
-        a.child = child.p;
-        a.f_rel_off = f_rel_off.p;
+        a.meta = meta.p;
+        a.cinfo = cinfo.p;
         a.rel = rel.p;
-        a.f_arrow_ptr = f_arrow_ptr.p;
         a.arrow_src = arrow_src.p;
         a.arrow_dst = arrow_dst.p;
-        a.f_inst = f_inst.p;
-        a.done = done;
-        a.fail = fail;
         a.slot_row = slot_row.p;
         a.slot_col = slot_col.p;
+        a.done = done;
+        a.fail = fail;
         return a;
     }
 };
 
-// Concatenate per-instance LU plans.  key_row / key_col map a slot key of
-// instance g to its global right-hand-side / solution index; ent_base[g] is
-// the offset of instance g's entries in the global value array.
+// Concatenate per-instance LU plans and renumber the fronts by (level, class).
+// key_row / key_col map a slot key of instance g to its global right-hand-side /
+// solution index; ent_base[g] is the offset of instance g's entries in the
+// global value array.
 template <typename KeyRow, typename KeyCol>
 void pack_fronts(std::vector<LUPlan const*> const& plans, std::vector<long long> const& ent_base,
                  KeyRow key_row, KeyCol key_col, long long n_index, long long n_values, DevFronts& d,
                  hipStream_t s) {
     int const G = static_cast<int>(plans.size());
-    std::vector<int> f_m, f_p, f_slot_ptr{0}, f_child_ptr{0}, child, f_rel_off{0}, rel, f_arrow_ptr{0},
-        arrow_src, arrow_dst, f_inst, slot_row, slot_col, f_level;
-    std::vector<long long> f_off{0}, f_voff{0};
-    int nlev = 0, fbase = 0;
+    // old (instance-concatenated) numbering
+    std::vector<int> o_inst, o_local, o_base(G + 1, 0);
+    int nlev = 0;
     d.max_front = 0;
     d.factor_entries = 0;
     for (int g = 0; g < G; ++g) {
-        LUPlan const& P = *plans[g];
-        int const slot0 = f_slot_ptr.back(), child0 = f_child_ptr.back(), rel0 = f_rel_off.back(),
-                  arr0 = f_arrow_ptr.back();
-        long long const off0 = f_off.back(), voff0 = f_voff.back();
-        for (int s2 = 0; s2 < P.n_fronts; ++s2) {
-            f_m.push_back(P.f_m[s2]);
-            f_p.push_back(P.f_p[s2]);
-            f_slot_ptr.push_back(slot0 + P.f_slot_ptr[s2 + 1]);
-            f_child_ptr.push_back(child0 + P.f_child_ptr[s2 + 1]);
-            f_rel_off.push_back(rel0 + P.f_rel_off[s2 + 1]);
-            f_arrow_ptr.push_back(arr0 + P.f_arrow_ptr[s2 + 1]);
-            f_off.push_back(off0 + P.f_off[s2 + 1]);
-            f_voff.push_back(voff0 + P.f_voff[s2 + 1]);
-            f_inst.push_back(g);
-            f_level.push_back(P.f_level[s2]);
-        }
-        for (int c : P.child) child.push_back(fbase + c);
-        rel.insert(rel.end(), P.rel.begin(), P.rel.end());
-        for (int q = 0; q < static_cast<int>(P.arrow_src.size()); ++q) {
-            arrow_src.push_back(static_cast<int>(ent_base[g] + P.arrow_src[q]));
-            arrow_dst.push_back(P.arrow_dst[q]);
-        }
-        for (int key : P.slot_key) {
-            slot_row.push_back(key_row(g, key));
-            slot_col.push_back(key_col(g, key));
-        }
-        nlev = std::max(nlev, P.n_levels);
-        fbase += P.n_fronts;
-        d.max_front = std::max(d.max_front, P.max_front);
-        d.factor_entries += P.factor_entries;
+        o_base[g + 1] = o_base[g] + plans[g]->n_fronts;
+        nlev = std::max(nlev, plans[g]->n_levels);
+        d.max_front = std::max(d.max_front, plans[g]->max_front);
+        d.factor_entries += plans[g]->factor_entries;
     }
-    d.n_fronts = fbase;
-    d.n_levels = nlev;
+    int const nf = o_base[G];
+    o_inst.resize(nf);
+    o_local.resize(nf);
+    for (int g = 0; g < G; ++g)
+        for (int k = 0; k < plans[g]->n_fronts; ++k) {
+            o_inst[o_base[g] + k] = g;
+            o_local[o_base[g] + k] = k;
+        }
+    auto M = [&](int o) { return plans[o_inst[o]]->f_m[o_local[o]]; };
+    auto Lv = [&](int o) { return plans[o_inst[o]]->f_level[o_local[o]]; };
+    for (int o = 0; o < nf; ++o)
+        if (M(o) > 65535) throw Error(GAPHIP_E_ARG, "front larger than 65535 slots");
     int const K = kNumClasses;
+    d.n_fronts = nf;
+    d.n_levels = nlev;
     d.cls_ptr.assign(static_cast<size_t>(nlev) * K + 1, 0);
-    for (int f = 0; f < fbase; ++f) {
-        if (f_m[f] > 65535) throw Error(GAPHIP_E_ARG, "front larger than 65535 slots");
-        d.cls_ptr[static_cast<size_t>(f_level[f]) * K + front_class(f_m[f]) + 1]++;
-    }
+    for (int o = 0; o < nf; ++o) d.cls_ptr[static_cast<size_t>(Lv(o)) * K + front_class(M(o)) + 1]++;
     for (size_t b = 0; b + 1 < d.cls_ptr.size(); ++b) d.cls_ptr[b + 1] += d.cls_ptr[b];
     d.level_ptr.assign(nlev + 1, 0);
     for (int l = 0; l <= nlev; ++l) d.level_ptr[l] = d.cls_ptr[static_cast<size_t>(l) * K];
-    std::vector<int> level_front(fbase);
+    std::vector<int> perm(nf), inv(nf);  // new -> old, old -> new (stable within a bucket)
+    {
+        std::vector<int> fp(d.cls_ptr.begin(), d.cls_ptr.end() - 1);
+        for (int o = 0; o < nf; ++o) {
+            int n = fp[static_cast<size_t>(Lv(o)) * K + front_class(M(o))]++;
+            perm[n] = o;
+            inv[o] = n;
+        }
+    }
+    std::vector<FrontMeta> meta(nf);
+    std::vector<ChildInfo> cinfo;
+    std::vector<int> rel_off_new(nf, 0), rel, arrow_src, arrow_dst, slot_row, slot_col;
+    long long off = 0, voff = 0;
+    for (int n = 0; n < nf; ++n) {
+        int const o = perm[n];
+        LUPlan const& P = *plans[o_inst[o]];
+        int const k = o_local[o];
+        FrontMeta& fm = meta[n];
+        fm.m = P.f_m[k];
+        fm.p = P.f_p[k];
+        fm.inst = o_inst[o];
+        fm.off = off;
+        fm.voff = voff;
+        off += static_cast<long long>(fm.m) * fm.m;
+        voff += fm.m;
+        fm.slot_beg = static_cast<int>(slot_row.size());
+        for (int t = P.f_slot_ptr[k]; t < P.f_slot_ptr[k + 1]; ++t) {
+            slot_row.push_back(key_row(fm.inst, P.slot_key[t]));
+            slot_col.push_back(key_col(fm.inst, P.slot_key[t]));
+        }
+        // arrowhead entries in row-major order of their targets
+        fm.arrow_beg = static_cast<int>(arrow_src.size());
+        std::vector<std::pair<int, int>> e;
+        for (int q = P.f_arrow_ptr[k]; q < P.f_arrow_ptr[k + 1]; ++q)
+            e.emplace_back(P.arrow_dst[q], static_cast<int>(ent_base[fm.inst] + P.arrow_src[q]));
+        std::sort(e.begin(), e.end());
+        for (auto const& pr : e) {
+            arrow_dst.push_back(pr.first);
+            arrow_src.push_back(pr.second);
+        }
+        fm.arrow_end = static_cast<int>(arrow_src.size());
+        // this front's extend-add map into its parent
+        rel_off_new[n] = static_cast<int>(rel.size());
+        for (int t = P.f_rel_off[k]; t < P.f_rel_off[k + 1]; ++t) rel.push_back(P.rel[t]);
+    }
+    d.front_entries = off;
+    d.vec_entries = voff;
+    for (int n = 0; n < nf; ++n) {
+        int const o = perm[n];
+        LUPlan const& P = *plans[o_inst[o]];
+        int const k = o_local[o];
+        FrontMeta& fm = meta[n];
+        fm.child_beg = static_cast<int>(cinfo.size());
+        for (int q = P.f_child_ptr[k]; q < P.f_child_ptr[k + 1]; ++q) {
+            int const cn = inv[o_base[o_inst[o]] + P.child[q]];
+            ChildInfo ci{};
+            ci.off = meta[cn].off;
+            ci.voff = meta[cn].voff;
+            ci.mc = meta[cn].m;
+            ci.pc = meta[cn].p;
+            ci.rel_off = rel_off_new[cn];
+            ci.id = cn;
+            if (cn >= n) throw Error(GAPHIP_E_ARG, "child must precede its parent");
+            cinfo.push_back(ci);
+        }
+        fm.child_end = static_cast<int>(cinfo.size());
+    }
+    // per (level, class) totals
     d.cls_mmax.assign(static_cast<size_t>(nlev) * K, 0);
     d.cls_pmax.assign(static_cast<size_t>(nlev) * K, 0);
     d.cls_arrow.assign(static_cast<size_t>(nlev) * K, 0);
     d.cls_factor.assign(static_cast<size_t>(nlev) * K, 0);
     d.cls_slots.assign(static_cast<size_t>(nlev) * K, 0);
-    {
-        std::vector<int> fp(d.cls_ptr.begin(), d.cls_ptr.end() - 1);
-        for (int f = 0; f < fbase; ++f) {
-            size_t b = static_cast<size_t>(f_level[f]) * K + front_class(f_m[f]);
-            level_front[fp[b]++] = f;
-            long long m = f_m[f], pp = f_p[f];
-            d.cls_arrow[b] += f_arrow_ptr[f + 1] - f_arrow_ptr[f];
+    for (size_t b = 0; b + 1 < d.cls_ptr.size(); ++b)
+        for (int n = d.cls_ptr[b]; n < d.cls_ptr[b + 1]; ++n) {
+            long long m = meta[n].m, pp = meta[n].p;
+            d.cls_arrow[b] += meta[n].arrow_end - meta[n].arrow_beg;
             d.cls_factor[b] += pp * m + (m - pp) * pp;
             d.cls_slots[b] += m;
             d.cls_mmax[b] = std::max<int>(d.cls_mmax[b], static_cast<int>(m));
             d.cls_pmax[b] = std::max<int>(d.cls_pmax[b], static_cast<int>(pp));
         }
-    }
-    // arrowhead entries of every front in row-major order of their targets
-    for (int f = 0; f < fbase; ++f) {
-        int const b0 = f_arrow_ptr[f], b1 = f_arrow_ptr[f + 1];
-        std::vector<std::pair<int, int>> e;
-        e.reserve(b1 - b0);
-        for (int q = b0; q < b1; ++q) e.emplace_back(arrow_dst[q], arrow_src[q]);
-        std::sort(e.begin(), e.end());
-        for (int q = b0; q < b1; ++q) {
-            arrow_dst[q] = e[q - b0].first;
-            arrow_src[q] = e[q - b0].second;
-        }
-    }
-    // per-row maps of the fronts handled by the blocked kernels
-    {
-        std::vector<int> bigbase(fbase, -1), arow, crow, crow_c, crow_a;
-        int rows = 0;
-        for (int f = 0; f < fbase; ++f) {
-            if (front_class(f_m[f]) != kNumClasses - 1) continue;
-            int const m = f_m[f];
-            bigbase[f] = rows;
-            rows += m + 1;
-            std::vector<int> na(m + 1, 0), nc(m + 1, 0);
-            for (int q = f_arrow_ptr[f]; q < f_arrow_ptr[f + 1]; ++q) na[(arrow_dst[q] >> 16) + 1]++;
-            for (int cq = f_child_ptr[f]; cq < f_child_ptr[f + 1]; ++cq) {
-                int c = child[cq];
-                for (int t = 0; t < f_m[c] - f_p[c]; ++t) nc[rel[f_rel_off[c] + t] + 1]++;
-            }
-            for (int r = 0; r < m; ++r) {
-                na[r + 1] += na[r];
-                nc[r + 1] += nc[r];
-            }
-            int const cbase = static_cast<int>(crow_c.size());
-            for (int r = 0; r <= m; ++r) {
-                arow.push_back(na[r]);
-                crow.push_back(cbase + nc[r]);
-            }
-            crow_c.resize(cbase + nc[m]);
-            crow_a.resize(cbase + nc[m]);
-            std::vector<int> fill(nc.begin(), nc.end() - 1);
-            for (int cq = f_child_ptr[f]; cq < f_child_ptr[f + 1]; ++cq) {
-                int c = child[cq];
-                for (int t = 0; t < f_m[c] - f_p[c]; ++t) {
-                    int r = rel[f_rel_off[c] + t];
-                    int at = cbase + fill[r]++;
-                    crow_c[at] = c;
-                    crow_a[at] = t;
-                }
-            }
-        }
-        if (arow.empty()) {
-            arow.push_back(0);
-            crow.push_back(0);
-        }
-        if (crow_c.empty()) {
-            crow_c.push_back(0);
-            crow_a.push_back(0);
-        }
-        d.bigbase.upload(bigbase, s);
-        d.arow.upload(arow, s);
-        d.crow.upload(crow, s);
-        d.crow_c.upload(crow_c, s);
-        d.crow_a.upload(crow_a, s);
-        HIPCHK(hipStreamSynchronize(s));
-    }
-    d.front_entries = f_off.back();
-    d.vec_entries = f_voff.back();
     // every index the front kernels dereference is checked here, on the host
-    {
-        long long const n_slots = static_cast<long long>(slot_row.size());
-        for (int f = 0; f < fbase; ++f) {
-            long long m = f_m[f], p = f_p[f];
-            if (m <= 0 || p <= 0 || p > m) throw Error(GAPHIP_E_ARG, "front with invalid size");
-            if (f_off[f + 1] - f_off[f] != m * m || f_voff[f + 1] - f_voff[f] != m)
-                throw Error(GAPHIP_E_ARG, "front offsets inconsistent");
-            if (f_slot_ptr[f + 1] - f_slot_ptr[f] != m || f_slot_ptr[f + 1] > n_slots)
-                throw Error(GAPHIP_E_ARG, "front slots inconsistent");
-            for (int q = f_arrow_ptr[f]; q < f_arrow_ptr[f + 1]; ++q)
-                if (arrow_dst[q] < 0 || (arrow_dst[q] >> 16) >= m || (arrow_dst[q] & 0xffff) >= m)
-                    throw Error(GAPHIP_E_ARG, "arrowhead map out of range");
-            for (int q = f_child_ptr[f]; q < f_child_ptr[f + 1]; ++q) {
-                int c = child[q];
-                if (c < 0 || c >= f) throw Error(GAPHIP_E_ARG, "child must precede its parent");
-                int nu = f_m[c] - f_p[c];
-                if (f_rel_off[c + 1] - f_rel_off[c] != nu) throw Error(GAPHIP_E_ARG, "extend-add map size");
-                for (int t = 0; t < nu; ++t)
-                    if (rel[f_rel_off[c] + t] < 0 || rel[f_rel_off[c] + t] >= m)
-                        throw Error(GAPHIP_E_ARG, "extend-add map out of range");
-            }
+    for (int n = 0; n < nf; ++n) {
+        FrontMeta const& fm = meta[n];
+        long long m = fm.m, p = fm.p;
+        if (m <= 0 || p <= 0 || p > m) throw Error(GAPHIP_E_ARG, "front with invalid size");
+        for (int q = fm.arrow_beg; q < fm.arrow_end; ++q) {
+            if ((arrow_dst[q] >> 16) >= m || (arrow_dst[q] & 0xffff) >= m)
+                throw Error(GAPHIP_E_ARG, "arrowhead map out of range");
+            if (arrow_src[q] < 0 || arrow_src[q] >= n_values) throw Error(GAPHIP_E_ARG, "arrowhead source out of range");
         }
-        for (long long q = 0; q < n_slots; ++q)
-            if (slot_row[q] < 0 || slot_col[q] < 0 || slot_row[q] >= n_index || slot_col[q] >= n_index)
-                throw Error(GAPHIP_E_ARG, "slot without equation/variable");
-        for (int src : arrow_src)
-            if (src < 0 || src >= n_values) throw Error(GAPHIP_E_ARG, "arrowhead source out of range");
-        if (static_cast<long long>(f_m.size()) != fbase || static_cast<int>(level_front.size()) != fbase)
-            throw Error(GAPHIP_E_ARG, "front count mismatch");
+        for (int q = fm.child_beg; q < fm.child_end; ++q) {
+            ChildInfo const& ci = cinfo[q];
+            for (int t = 0; t < ci.mc - ci.pc; ++t)
+                if (rel[ci.rel_off + t] < 0 || rel[ci.rel_off + t] >= m)
+                    throw Error(GAPHIP_E_ARG, "extend-add map out of range");
+        }
     }
-    d.level_front.upload(level_front, s);
-    d.f_m.upload(f_m, s);
-    d.f_p.upload(f_p, s);
-    d.f_slot_ptr.upload(f_slot_ptr, s);
-    d.f_child_ptr.upload(f_child_ptr, s);
-    d.child.upload(child, s);
-    d.f_rel_off.upload(f_rel_off, s);
+    for (size_t q = 0; q < slot_row.size(); ++q)
+        if (slot_row[q] < 0 || slot_col[q] < 0 || slot_row[q] >= n_index || slot_col[q] >= n_index)
+            throw Error(GAPHIP_E_ARG, "slot without equation/variable");
+    // per-row maps of the fronts handled by the blocked kernels: row pointers
+    // into the arrowhead list and into (child info index, child row) pairs
+    std::vector<int> bigbase(nf, -1), arow, crow, crow_ci;
+    int rows = 0, nent = 0;
+    for (int n = 0; n < nf; ++n) {
+        FrontMeta const& fm = meta[n];
+        if (front_class(fm.m) != kNumClasses - 1) continue;
+        int const m = fm.m;
+        bigbase[n] = rows;
+        rows += m + 1;
+        std::vector<int> na(m + 1, 0), nc(m + 1, 0);
+        for (int q = fm.arrow_beg; q < fm.arrow_end; ++q) na[(arrow_dst[q] >> 16) + 1]++;
+        for (int q = fm.child_beg; q < fm.child_end; ++q)
+            for (int t = 0; t < cinfo[q].mc - cinfo[q].pc; ++t) nc[rel[cinfo[q].rel_off + t] + 1]++;
+        for (int r = 0; r < m; ++r) {
+            na[r + 1] += na[r];
+            nc[r + 1] += nc[r];
+        }
+        for (int r = 0; r <= m; ++r) {
+            arow.push_back(na[r]);
+            crow.push_back(nent + nc[r]);
+        }
+        crow_ci.resize(2 * static_cast<size_t>(nent + nc[m]));
+        std::vector<int> fill(nc.begin(), nc.end() - 1);
+        for (int q = fm.child_beg; q < fm.child_end; ++q)
+            for (int t = 0; t < cinfo[q].mc - cinfo[q].pc; ++t) {
+                int r = rel[cinfo[q].rel_off + t];
+                int at = nent + fill[r]++;
+                crow_ci[2 * static_cast<size_t>(at)] = q;
+                crow_ci[2 * static_cast<size_t>(at) + 1] = t;
+            }
+        nent += nc[m];
+    }
+    if (arow.empty()) {
+        arow.push_back(0);
+        crow.push_back(0);
+    }
+    if (crow_ci.empty()) crow_ci.assign(2, 0);
+    if (cinfo.empty()) cinfo.push_back(ChildInfo{});
+    if (rel.empty()) rel.push_back(0);
+    d.meta.upload(meta, s);
+    d.cinfo.upload(cinfo, s);
     d.rel.upload(rel, s);
-    d.f_arrow_ptr.upload(f_arrow_ptr, s);
     d.arrow_src.upload(arrow_src, s);
     d.arrow_dst.upload(arrow_dst, s);
-    d.f_inst.upload(f_inst, s);
     d.slot_row.upload(slot_row, s);
     d.slot_col.upload(slot_col, s);
-    d.f_off.upload(f_off, s);
-    d.f_voff.upload(f_voff, s);
+    d.bigbase.upload(bigbase, s);
+    d.arow.upload(arow, s);
+    d.crow.upload(crow, s);
+    d.crow_ci.upload(crow_ci, s);
     HIPCHK(hipStreamSynchronize(s));  // host vectors die at return
 }
 
@@ -451,17 +451,19 @@ struct FrontRunner {
                 if (!nb) continue;
                 double bytes = (w + 4) * d.cls_arrow[b] + w * d.cls_factor[b];
                 hook(kid + c, bytes, [&] {
+                    int const gb = c < kNumClasses - 1 ? blocks_for(nb, kClassPerBlock[c]) : 0;
                     switch (c) {
-                        case 0: hipLaunchKernelGGL((k_front_factor_lds<T, 16, 64>), dim3(nb), dim3(64), 0, s, a, beg, jv, front); break;
-                        case 1: hipLaunchKernelGGL((k_front_factor_lds<T, 32, 64>), dim3(nb), dim3(64), 0, s, a, beg, jv, front); break;
-                        case 2: hipLaunchKernelGGL((k_front_factor_lds<T, 64, 64>), dim3(nb), dim3(64), 0, s, a, beg, jv, front); break;
-                        case 3: hipLaunchKernelGGL((k_front_factor_lds<T, 128, 128>), dim3(nb), dim3(128), 0, s, a, beg, jv, front); break;
+                        case 0: hipLaunchKernelGGL((k_front_factor_lds<T, 8, 8, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
+                        case 1: hipLaunchKernelGGL((k_front_factor_lds<T, 16, 16, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
+                        case 2: hipLaunchKernelGGL((k_front_factor_lds<T, 32, 32, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
+                        case 3: hipLaunchKernelGGL((k_front_factor_lds<T, 64, 64, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
+                        case 4: hipLaunchKernelGGL((k_front_factor_lds<T, 128, 128, 128>), dim3(gb), dim3(128), 0, s, a, beg, nb, jv, front); break;
                         default: big_factor(d, a, beg, nb, d.cls_mmax[b], d.cls_pmax[b], jv, front, s);
                     }
                 });
             }
     }
-    template <typename Hook>
+    template <bool TR = false, typename Hook>
     static void forward(DevFronts const& d, FrontArgs const& a, T const* front, T* vec, double const* rhs,
                         double sign, hipStream_t s, int kid, Hook&& hook) {
         double const w = sizeof(T);
@@ -472,17 +474,19 @@ struct FrontRunner {
                 if (!nb) continue;
                 double bytes = w * d.cls_factor[b] / 2 + (2 * w + 12) * d.cls_slots[b];
                 hook(kid + c, bytes, [&] {
+                    int const gb = c < kNumClasses - 1 ? blocks_for(nb, kClassPerBlock[c]) : 0;
                     switch (c) {
-                        case 0: hipLaunchKernelGGL((k_front_fwd_lds<T, 16, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, rhs, sign); break;
-                        case 1: hipLaunchKernelGGL((k_front_fwd_lds<T, 32, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, rhs, sign); break;
-                        case 2: hipLaunchKernelGGL((k_front_fwd_lds<T, 64, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, rhs, sign); break;
-                        case 3: hipLaunchKernelGGL((k_front_fwd_lds<T, 128, 128>), dim3(nb), dim3(128), 0, s, a, beg, front, vec, rhs, sign); break;
-                        default: hipLaunchKernelGGL(k_front_fwd<T>, dim3(nb), dim3(256), 0, s, a, beg, front, vec, rhs, sign);
+                        case 0: hipLaunchKernelGGL((k_front_fwd_lds<T, 8, 8, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        case 1: hipLaunchKernelGGL((k_front_fwd_lds<T, 16, 16, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        case 2: hipLaunchKernelGGL((k_front_fwd_lds<T, 32, 32, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        case 3: hipLaunchKernelGGL((k_front_fwd_lds<T, 64, 64, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        case 4: hipLaunchKernelGGL((k_front_fwd_lds<T, 128, 128, 128, TR>), dim3(gb), dim3(128), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        default: hipLaunchKernelGGL((k_front_fwd<T, TR>), dim3(nb), dim3(256), 0, s, a, beg, front, vec, rhs, sign);
                     }
                 });
             }
     }
-    template <typename Hook>
+    template <bool TR = false, typename Hook>
     static void backward(DevFronts const& d, FrontArgs const& a, T const* front, T* vec, double* sol,
                          hipStream_t s, int kid, Hook&& hook) {
         double const w = sizeof(T);
@@ -493,12 +497,14 @@ struct FrontRunner {
                 if (!nb) continue;
                 double bytes = w * d.cls_factor[b] / 2 + (2 * w + 12) * d.cls_slots[b];
                 hook(kid + c, bytes, [&] {
+                    int const gb = c < kNumClasses - 1 ? blocks_for(nb, kClassPerBlock[c]) : 0;
                     switch (c) {
-                        case 0: hipLaunchKernelGGL((k_front_bwd_lds<T, 16, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, sol); break;
-                        case 1: hipLaunchKernelGGL((k_front_bwd_lds<T, 32, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, sol); break;
-                        case 2: hipLaunchKernelGGL((k_front_bwd_lds<T, 64, 64>), dim3(nb), dim3(64), 0, s, a, beg, front, vec, sol); break;
-                        case 3: hipLaunchKernelGGL((k_front_bwd_lds<T, 128, 128>), dim3(nb), dim3(128), 0, s, a, beg, front, vec, sol); break;
-                        default: hipLaunchKernelGGL(k_front_bwd<T>, dim3(nb), dim3(256), 0, s, a, beg, front, vec, sol);
+                        case 0: hipLaunchKernelGGL((k_front_bwd_lds<T, 8, 8, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
+                        case 1: hipLaunchKernelGGL((k_front_bwd_lds<T, 16, 16, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
+                        case 2: hipLaunchKernelGGL((k_front_bwd_lds<T, 32, 32, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
+                        case 3: hipLaunchKernelGGL((k_front_bwd_lds<T, 64, 64, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
+                        case 4: hipLaunchKernelGGL((k_front_bwd_lds<T, 128, 128, 128, TR>), dim3(gb), dim3(128), 0, s, a, beg, nb, front, vec, sol); break;
+                        default: hipLaunchKernelGGL((k_front_bwd<T, TR>), dim3(nb), dim3(256), 0, s, a, beg, front, vec, sol);
                     }
                 });
             }
@@ -579,6 +585,9 @@ struct gaphip_plan {
         it_pos, done, iters, lu_fail, active;
     gaphip::DBuf<double> flat, br_r, br_x, br_g, br_b, app_g, app_b, zb, f, dx, final_mis, jv64;
     gaphip::DBuf<float> jv32;
+    gaphip::DBuf<int> br_f, br_t, br_inst;  // in-service branch endpoints (global bus) or -1
+    gaphip::DBuf<int> bus_type_d;
+    gaphip::DBuf<double> adj_lossb, adj_gp, adj_gq, adj_gr, adj_gx, adj_gvm, adj_gva;
     gaphip::DBuf<double2> v, vpol, s, sspec, y_pu, y_given;
     std::vector<int> y_given_mode;  // per instance: 0 assemble, 1 siemens given, 2 per-unit given
     gaphip::DBuf<unsigned long long> maxf;
@@ -788,6 +797,34 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     P.it_j.upload(it_j, s);
     P.it_y.upload(it_y, s);
     P.it_pos.upload(it_pos, s);
+    {
+        // branch endpoints for the adjoint (out-of-service / unknown ids -> -1)
+        std::vector<int> bf(std::max<long long>(P.n_br, 1), -1), bt(std::max<long long>(P.n_br, 1), -1),
+            bi(std::max<long long>(P.n_br, 1), 0);
+        for (int g = 0; g < n_inst; ++g) {
+            auto const& N = P.nets[g];
+            auto const& Y = P.yp[g];
+            auto lookup = [&](int id) {
+                return (id >= 0 && id < static_cast<int>(Y.id2idx.size())) ? Y.id2idx[id] : -1;
+            };
+            for (int k = 0; k < N.n_branch(); ++k) {
+                int const gk = P.br_off[g] + k;
+                bi[gk] = g;
+                if (!N.br_status[k]) continue;
+                int f = lookup(N.br_from[k]), t = lookup(N.br_to[k]);
+                if (f < 0 || t < 0) continue;
+                bf[gk] = P.bus_off[g] + f;
+                bt[gk] = P.bus_off[g] + t;
+            }
+        }
+        P.br_f.upload(bf, s);
+        P.br_t.upload(bt, s);
+        P.br_inst.upload(bi, s);
+        std::vector<int> btype(P.n_bus);
+        for (int g = 0; g < n_inst; ++g)
+            for (int i = 0; i < P.nets[g].n_bus(); ++i) btype[P.bus_off[g] + i] = P.nets[g].bus_type[i];
+        P.bus_type_d.upload(btype, s);
+    }
     HIPCHK(hipStreamSynchronize(s));
 
     // fronts
@@ -968,6 +1005,88 @@ void run_solve(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front, T* v
     if (P.prof.on) P.prof.collect();
     P.have_solution = true;
     P.last_base = o.base_power;
+}
+
+// Adjoint at the converged state of the last solve: refactor J(x*) in FP64,
+// solve J^T lambda = dL/dx with the transposed front solves, then the bus and
+// branch gradient kernels.
+void run_adjoint(gaphip_plan& P, double w_loss, double w_dev, double const* g_vm, double const* g_va,
+                 double* gp, double* gq, double* gr, double* gx, double* loss_out) {
+    if (!P.have_solution) throw Error(GAPHIP_E_STATE, "adjoint needs a converged solve first");
+    hipStream_t s = P.stream;
+    int const G = P.n_inst;
+    long long const N = P.n_bus;
+    if (!P.jv64.p) P.jv64.alloc(std::max<long long>(P.j_nnz, 1));
+    if (!P.front64.p) P.front64.alloc(std::max<long long>(P.fr.front_entries, 1));
+    if (!P.vec64.p) P.vec64.alloc(std::max<long long>(P.fr.vec_entries, 1));
+    P.adj_lossb.alloc(N);
+    P.adj_gp.alloc(N);
+    P.adj_gq.alloc(N);
+    P.adj_gr.alloc(std::max<long long>(P.n_br, 1));
+    P.adj_gx.alloc(std::max<long long>(P.n_br, 1));
+    double const* dgvm = nullptr;
+    double const* dgva = nullptr;
+    if (g_vm) {
+        P.adj_gvm.alloc(N);
+        HIPCHK(hipMemcpyAsync(P.adj_gvm.p, g_vm, sizeof(double) * N, hipMemcpyHostToDevice, s));
+        dgvm = P.adj_gvm.p;
+    }
+    if (g_va) {
+        P.adj_gva.alloc(N);
+        HIPCHK(hipMemcpyAsync(P.adj_gva.p, g_va, sizeof(double) * N, hipMemcpyHostToDevice, s));
+        dgva = P.adj_gva.p;
+    }
+    HIPCHK(hipMemsetAsync(P.done.p, 0, sizeof(int) * G, s));
+    HIPCHK(hipMemsetAsync(P.lu_fail.p, 0, sizeof(int) * G, s));
+    if (P.maxf_rows < 1) {
+        P.maxf.alloc(G);
+        P.active.alloc(1);
+        P.maxf_rows = 1;
+    }
+    // S, |V|, arg V at the solution, then J(x*) and its FP64 factors
+    MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
+               P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p};
+    hipLaunchKernelGGL(k_mismatch, dim3(blocks_for(N, 256)), dim3(256), 0, s, ma);
+    JacArgs ja{static_cast<int>(P.n_items), P.it_i.p, P.it_j.p, P.it_y.p, P.it_pos.p, P.bus_inst.p, P.done.p,
+               P.y_pu.p, P.s.p, P.vpol.p};
+    hipLaunchKernelGGL(k_jacobian<double>, dim3(blocks_for(P.n_items, 256)), dim3(256), 0, s, ja, P.jv64.p);
+    FrontArgs fra = P.fr.args(P.done.p, P.lu_fail.p);
+    FrontRunner<double>::factor(P.fr, fra, P.jv64.p, P.front64.p, s, 0, PlainHook{});
+    // dL/dx into f (per variable), loss contributions per bus
+    HIPCHK(hipMemsetAsync(P.f.p, 0, sizeof(double) * std::max<long long>(P.n_var, 1), s));
+    AdjArgs aa{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.s.p, P.ang.p, P.mag.p, P.bus_type_d.p,
+               dgvm, dgva, w_loss, w_dev, P.f.p, P.adj_lossb.p};
+    hipLaunchKernelGGL(k_adj_rhs, dim3(blocks_for(N, 256)), dim3(256), 0, s, aa);
+    // J^T lambda = dL/dx: U^T then L^T; lambda lands in dx, indexed by equation
+    FrontRunner<double>::forward<true>(P.fr, fra, P.front64.p, P.vec64.p, P.f.p, 1.0, s, 0, PlainHook{});
+    FrontRunner<double>::backward<true>(P.fr, fra, P.front64.p, P.vec64.p, P.dx.p, s, 0, PlainHook{});
+    AdjBusArgs ba{static_cast<int>(N), P.peq.p, P.qeq.p, P.bus_inst.p, P.dx.p, 1.0 / P.last_base, P.adj_gp.p,
+                  P.adj_gq.p};
+    hipLaunchKernelGGL(k_adj_bus, dim3(blocks_for(N, 256)), dim3(256), 0, s, ba);
+    if (P.n_br > 0) {
+        AdjBrArgs bra{static_cast<int>(P.n_br), P.br_f.p, P.br_t.p, P.br_inst.p, P.br_r.p, P.br_x.p, P.zb.p, P.v.p,
+                      P.peq.p, P.qeq.p, P.dx.p, w_loss, P.adj_gr.p, P.adj_gx.p};
+        hipLaunchKernelGGL(k_adj_branch, dim3(blocks_for(P.n_br, 256)), dim3(256), 0, s, bra);
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<int> fail(G);
+    HIPCHK(hipMemcpyAsync(fail.data(), P.lu_fail.p, sizeof(int) * G, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int g = 0; g < G; ++g)
+        if (fail[g]) throw Error(GAPHIP_E_SINGULAR, "Jacobian at the solution of network " + std::to_string(g) + " is singular");
+    if (gp) HIPCHK(hipMemcpy(gp, P.adj_gp.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (gq) HIPCHK(hipMemcpy(gq, P.adj_gq.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (gr && P.n_br) HIPCHK(hipMemcpy(gr, P.adj_gr.p, sizeof(double) * P.n_br, hipMemcpyDeviceToHost));
+    if (gx && P.n_br) HIPCHK(hipMemcpy(gx, P.adj_gx.p, sizeof(double) * P.n_br, hipMemcpyDeviceToHost));
+    if (loss_out) {
+        std::vector<double> lb(N);
+        HIPCHK(hipMemcpy(lb.data(), P.adj_lossb.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+        for (int g = 0; g < G; ++g) {
+            double acc = 0.0;
+            for (int i = P.bus_off[g]; i < P.bus_off[g + 1]; ++i) acc += lb[i];
+            loss_out[g] = acc;
+        }
+    }
 }
 
 }  // namespace gaphip
@@ -1464,10 +1583,10 @@ int gaphip_lu_solve(gaphip_lu* L, const double* rhs_ri, double* x_ri, int n) {
 
 }  // extern "C"
 
-extern "C" int gaphip_adjoint(gaphip_plan* P, double, double, const double*, const double*, double*, double*,
-                              double*, double*, double*) {
+extern "C" int gaphip_adjoint(gaphip_plan* P, double w_loss, double w_dev, const double* g_vm, const double* g_va,
+                              double* grad_p, double* grad_q, double* grad_r, double* grad_x, double* loss_out) {
     GAPHIP_GUARD({
         if (!P) throw Error(GAPHIP_E_ARG, "null plan");
-        throw Error(GAPHIP_E_STATE, "adjoint is not available in this build");
+        run_adjoint(*P, w_loss, w_dev, g_vm, g_va, grad_p, grad_q, grad_r, grad_x, loss_out);
     })
 }

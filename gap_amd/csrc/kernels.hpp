@@ -245,26 +245,32 @@ __global__ void __launch_bounds__(256) k_jacobian(JacArgs a, T* jv) {
 // structure slots shared with ancestors.  Factorisation leaves L (unit,
 // below the diagonal) and U in the first p rows / columns and the Schur
 // complement in the trailing block, which the parent reads during assembly.
-// Arrowhead targets are packed as (row << 16) | col.
+// Arrowhead targets are packed as (row << 16) | col.  Fronts are numbered in
+// (level, size class) order, so a launch covers ids [begin, begin + count).
+struct alignas(16) FrontMeta {
+    long long off;   // front storage offset (m*m entries)
+    long long voff;  // slot-vector offset (m entries)
+    int m, p;
+    int child_beg, child_end;  // ChildInfo range
+    int arrow_beg, arrow_end;  // arrowhead entries
+    int slot_beg, inst;
+};
+struct alignas(16) ChildInfo {
+    long long off, voff;  // the child's front / vector offsets
+    int mc, pc;           // its size and pivot count
+    int rel_off, id;      // its extend-add map into this front; its front id
+};
+
 struct FrontArgs {
-    const int* level_front;
-    const int* f_m;
-    const int* f_p;
-    const long long* f_off;
-    const long long* f_voff;
-    const int* f_slot_ptr;
-    const int* f_child_ptr;
-    const int* child;
-    const int* f_rel_off;
+    const FrontMeta* meta;
+    const ChildInfo* cinfo;
     const int* rel;
-    const int* f_arrow_ptr;
     const int* arrow_src;
     const int* arrow_dst;
-    const int* f_inst;
-    const int* done;  // may be null
-    int* fail;        // per instance; may be null
     const int* slot_row;  // per slot: index into the right-hand side
     const int* slot_col;  // per slot: index into the solution
+    const int* done;      // per instance; may be null
+    int* fail;            // per instance; may be null
 };
 
 template <typename T>
@@ -272,58 +278,127 @@ __device__ inline T tabs(T x) {
     return x < T(0) ? -x : x;
 }
 
-// Fronts of up to MM slots, one workgroup (NT threads) per front, the whole
-// front staged in LDS (leading dimension MM + 1).  Thread t owns column t
-// (t < MM), so every pivot step is one pass and one barrier.
-template <typename T, int MM, int NT>
-__global__ void __launch_bounds__(NT) k_front_factor_lds(FrontArgs a, int begin, const T* jv, T* front) {
-    static_assert(NT >= MM, "one thread per column");
-    constexpr int LD = MM + 1;
-    __shared__ T F[MM * LD];
-    int const s = a.level_front[begin + blockIdx.x];
-    int const inst = a.f_inst[s];
-    if (a.done && a.done[inst]) return;
-    int const m = a.f_m[s], p = a.f_p[s];
-    int const t = threadIdx.x;
-    for (int e = t; e < m * LD; e += NT) F[e] = T(0);
+// all lanes of a group (L lanes) hold the same value; fold across the groups
+// of the workgroup so loop trip counts are uniform and barriers legal
+template <int L, int NT>
+__device__ inline int group_max(int v) {
+    for (int off = L; off < NT && off < kWave; off <<= 1) v = max(v, __shfl_xor(v, off));
+    if constexpr (NT > kWave) {
+        __shared__ int red[NT / kWave];
+        int const w = threadIdx.x / kWave;
+        if ((threadIdx.x & (kWave - 1)) == 0) red[w] = v;
+        __syncthreads();
+        v = red[0];
+        for (int k = 1; k < NT / kWave; ++k) v = max(v, red[k]);
+        __syncthreads();
+    }
+    return v;
+}
+
+// rows [0, n) of a row-major global block (leading dimension ld) into the LDS
+// tile F (leading dimension LD), lane t copying column t; CH independent loads
+// are issued before their LDS stores
+template <typename T, int MM, int L, int CH>
+__device__ inline void load_rows(T* F, int LD, const T* G, int ld, int n, int t, bool active) {
+    for (int r0 = 0; r0 < n; r0 += CH) {
+        T v[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) v[u] = (active && r0 + u < n) ? G[static_cast<long long>(r0 + u) * ld + t] : T(0);
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+            if (active && r0 + u < n) F[(r0 + u) * LD + t] = v[u];
+    }
+}
+
+// F[rel[r], rel[t]] += child update block (nu x nu); the block rows are loaded
+// CH at a time.  The map entry of row r comes from lane r of the group when a
+// group fits in one wavefront (shuffles do not cross wavefronts), otherwise
+// from memory.
+template <typename T, int L, int CH>
+__device__ inline void extend_add(T* F, int LD, const T* Fc, int mc, int nu, const int* rl, int t) {
+    int const myrl = t < nu ? rl[t] : 0;
+    for (int r0 = 0; r0 < nu; r0 += CH) {
+        T v[CH];
+        int rr[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            v[u] = (t < nu && r0 + u < nu) ? Fc[static_cast<long long>(r0 + u) * mc + t] : T(0);
+            if constexpr (L <= kWave)
+                rr[u] = __shfl(myrl, (r0 + u) & (L - 1), L);
+            else
+                rr[u] = (r0 + u < nu) ? rl[r0 + u] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+            if (t < nu && r0 + u < nu) F[rr[u] * LD + myrl] += v[u];
+    }
+}
+
+// Small fronts: G = NT / L fronts per workgroup, L lanes per front (L >= MM),
+// each front staged in its own LDS tile (leading dimension MM + 1).  Lane t of a
+// group owns column t.  Loop trip counts are made uniform over the workgroup
+// (max over its fronts) so every barrier is reached by all threads; a front
+// that is finished, inactive or converged only idles.
+template <typename T, int MM, int L, int NT>
+__global__ void __launch_bounds__(NT) k_front_factor_lds(FrontArgs a, int begin, int count, const T* jv,
+                                                          T* front) {
+    static_assert(L >= MM && NT % L == 0, "one lane per column");
+    constexpr int G = NT / L, LD = MM + 1, CH = MM < 8 ? MM : 8;
+    __shared__ T Fs[G][MM * LD];
+    int const g = threadIdx.x / L, t = threadIdx.x % L;
+    int const idx = blockIdx.x * G + g;
+    FrontMeta fm{};
+    if (idx < count) {
+        fm = a.meta[begin + idx];
+        if (a.done && a.done[fm.inst]) fm.m = fm.p = fm.child_end = fm.child_beg = fm.arrow_end = fm.arrow_beg = 0;
+    }
+    int const m = fm.m, p = fm.p;
+    T* F = Fs[g];
+    for (int e = t; e < m * LD; e += L) F[e] = T(0);
     __syncthreads();
-    for (int q = a.f_arrow_ptr[s] + t; q < a.f_arrow_ptr[s + 1]; q += NT) {
-        int d = a.arrow_dst[q];
-        F[(d >> 16) * LD + (d & 0xffff)] = jv[a.arrow_src[q]];
+    for (int q0 = fm.arrow_beg; q0 < fm.arrow_end; q0 += L) {
+        int const q = q0 + t;
+        if (q < fm.arrow_end) {
+            int const d = a.arrow_dst[q];
+            F[(d >> 16) * LD + (d & 0xffff)] = jv[a.arrow_src[q]];
+        }
     }
     __syncthreads();
-    for (int cq = a.f_child_ptr[s]; cq < a.f_child_ptr[s + 1]; ++cq) {
-        int const c = a.child[cq];
-        int const mc = a.f_m[c], pc = a.f_p[c], nu = mc - pc;
-        const T* Fc = front + a.f_off[c] + static_cast<long long>(pc) * mc + pc;
-        const int* rl = a.rel + a.f_rel_off[c];
-        if (t < nu) {
-            int const col = rl[t];
-            for (int r = 0; r < nu; ++r) F[rl[r] * LD + col] += Fc[r * mc + t];
+    int const nch = fm.child_end - fm.child_beg;
+    int const nch_max = group_max<L, NT>(nch);
+    for (int ci = 0; ci < nch_max; ++ci) {
+        if (ci < nch) {
+            ChildInfo const c = a.cinfo[fm.child_beg + ci];
+            int const nu = c.mc - c.pc;
+            extend_add<T, L, CH>(F, LD, front + c.off + static_cast<long long>(c.pc) * c.mc + c.pc, c.mc, nu,
+                                 a.rel + c.rel_off, t);
         }
         __syncthreads();
     }
-    // right-looking elimination of the p pivots; column k is scaled by its
-    // owner after the barrier that closes step k (no later step reads it)
-    for (int k = 0; k < p; ++k) {
-        T piv = F[k * LD + k];
-        if (tabs(piv) < T(1e-14)) {
-            if (t == 0 && a.fail) a.fail[inst] = 1;
+    // right-looking elimination; column k is scaled by its owner after the
+    // barrier that closes step k (no later step reads it)
+    int const p_max = group_max<L, NT>(p);
+    for (int k = 0; k < p_max; ++k) {
+        T piv = k < p ? F[k * LD + k] : T(1);
+        bool const tiny = tabs(piv) < T(1e-14);
+        if (tiny) {
+            if (k < p && t == 0 && a.fail) a.fail[fm.inst] = 1;
             piv = T(1);
         }
         T const rp = T(1) / piv;
-        if (t > k && t < m) {
+        if (k < p && t > k && t < m) {
             T const ukj = F[k * LD + t];
             for (int i = k + 1; i < m; ++i) F[i * LD + t] -= (F[i * LD + k] * rp) * ukj;
         }
         __syncthreads();
-        if (t == k)
+        if (k < p && t == k)
             for (int i = k + 1; i < m; ++i) F[i * LD + k] *= rp;
     }
     __syncthreads();
-    T* G = front + a.f_off[s];
-    for (int r = 0; r < m; ++r)
-        if (t < m) G[r * m + t] = F[r * LD + t];
+    if (m > 0 && t < m) {
+        T* Gm = front + fm.off;
+        for (int r = 0; r < m; ++r) Gm[r * m + t] = F[r * LD + t];
+    }
 }
 
 // ---------------------------------------------------- large fronts (> 128)
@@ -332,38 +407,37 @@ __global__ void __launch_bounds__(NT) k_front_factor_lds(FrontArgs a, int begin,
 // a triangular solve for the U12 block and a trailing GEMM update.  All fronts
 // of one (level, class) group advance together; a front whose panels are done
 // simply exits.  Per-row maps (host-built): arrowhead entries sorted by row
-// (arow) and the child update rows that land on each parent row (crow_c /
-// crow_a), listed in child order so sums match the small-front kernels.
+// (arow) and the child update rows that land on each parent row (crow_ci as
+// (child info index, child row) pairs), listed in child order so sums match
+// the small-front kernels.
 struct BigMaps {
     const int* bigbase;  // per front: offset of its m+1 row pointers
-    const int* arow;     // row r of front f: arrowhead entries [arow[b+r], arow[b+r+1]) after f_arrow_ptr[f]
+    const int* arow;     // row r of front f: arrowhead entries [arow[b+r], arow[b+r+1]) after arrow_beg
     const int* crow;     // row r: child rows [crow[b+r], crow[b+r+1])
-    const int* crow_c;   // child front of each child row
-    const int* crow_a;   // row index inside that child's update block
+    const int* crow_ci;  // pairs (child info index, row inside that child's update block)
 };
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_assemble(FrontArgs a, BigMaps b, int begin, const T* jv, T* front) {
-    int const s = a.level_front[begin + blockIdx.y];
-    int const inst = a.f_inst[s];
-    if (a.done && a.done[inst]) return;
-    int const m = a.f_m[s];
+    FrontMeta const fm = a.meta[begin + blockIdx.y];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m;
     int const r = blockIdx.x;
     if (r >= m) return;
     int const tid = threadIdx.x;
-    T* row = front + a.f_off[s] + static_cast<long long>(r) * m;
+    T* row = front + fm.off + static_cast<long long>(r) * m;
     for (int c = tid; c < m; c += 256) row[c] = T(0);
     __syncthreads();
-    int const base = b.bigbase[s];
-    int const a0 = a.f_arrow_ptr[s];
-    for (int q = a0 + b.arow[base + r] + tid; q < a0 + b.arow[base + r + 1]; q += 256)
+    int const base = b.bigbase[begin + blockIdx.y];
+    for (int q = fm.arrow_beg + b.arow[base + r] + tid; q < fm.arrow_beg + b.arow[base + r + 1]; q += 256)
         row[a.arrow_dst[q] & 0xffff] = jv[a.arrow_src[q]];
     __syncthreads();
     for (int e = b.crow[base + r]; e < b.crow[base + r + 1]; ++e) {
-        int const c = b.crow_c[e], ra = b.crow_a[e];
-        int const mc = a.f_m[c], pc = a.f_p[c], nu = mc - pc;
-        const T* src = front + a.f_off[c] + static_cast<long long>(pc + ra) * mc + pc;
-        const int* rl = a.rel + a.f_rel_off[c];
+        ChildInfo const c = a.cinfo[b.crow_ci[2 * e]];
+        int const ra = b.crow_ci[2 * e + 1];
+        int const nu = c.mc - c.pc;
+        const T* src = front + c.off + static_cast<long long>(c.pc + ra) * c.mc + c.pc;
+        const int* rl = a.rel + c.rel_off;
         for (int t = tid; t < nu; t += 256) row[rl[t]] += src[t];
         __syncthreads();
     }
@@ -371,18 +445,17 @@ __global__ void __launch_bounds__(256) k_big_assemble(FrontArgs a, BigMaps b, in
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_panel(FrontArgs a, int begin, int k0, int nb, T* front) {
-    int const s = a.level_front[begin + blockIdx.x];
-    int const inst = a.f_inst[s];
-    if (a.done && a.done[inst]) return;
-    int const m = a.f_m[s], p = a.f_p[s];
+    FrontMeta const fm = a.meta[begin + blockIdx.x];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
     if (k0 >= p) return;
     int const kend = min(k0 + nb, p);
     int const tid = threadIdx.x;
-    T* F = front + a.f_off[s];
+    T* F = front + fm.off;
     for (int k = k0; k < kend; ++k) {
         T piv = F[static_cast<long long>(k) * m + k];
         if (tabs(piv) < T(1e-14)) {
-            if (tid == 0 && a.fail) a.fail[inst] = 1;
+            if (tid == 0 && a.fail) a.fail[fm.inst] = 1;
             piv = T(1);
         }
         T const rp = T(1) / piv;
@@ -399,15 +472,14 @@ __global__ void __launch_bounds__(256) k_big_panel(FrontArgs a, int begin, int k
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_trsm(FrontArgs a, int begin, int k0, int nb, T* front) {
-    int const s = a.level_front[begin + blockIdx.y];
-    int const inst = a.f_inst[s];
-    if (a.done && a.done[inst]) return;
-    int const m = a.f_m[s], p = a.f_p[s];
+    FrontMeta const fm = a.meta[begin + blockIdx.y];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
     if (k0 >= p) return;
     int const kend = min(k0 + nb, p);
     int const j = kend + blockIdx.x * 256 + threadIdx.x;
     if (j >= m) return;
-    T* F = front + a.f_off[s];
+    T* F = front + fm.off;
     for (int r = k0 + 1; r < kend; ++r) {
         T acc = F[static_cast<long long>(r) * m + j];
         const T* lr = F + static_cast<long long>(r) * m;
@@ -423,16 +495,15 @@ __global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0
     constexpr int TB = 64, KB = 32;
     __shared__ T As[KB][TB + 1];
     __shared__ T Bs[KB][TB + 1];
-    int const s = a.level_front[begin + blockIdx.z];
-    int const inst = a.f_inst[s];
-    if (a.done && a.done[inst]) return;
-    int const m = a.f_m[s], p = a.f_p[s];
+    FrontMeta const fm = a.meta[begin + blockIdx.z];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
     if (k0 >= p) return;
     int const kend = min(k0 + nb, p);
     int const kb = kend - k0;
     int const i0 = kend + blockIdx.y * TB, j0 = kend + blockIdx.x * TB;
     if (i0 >= m || j0 >= m) return;
-    T* F = front + a.f_off[s];
+    T* F = front + fm.off;
     int const tid = threadIdx.x;
     for (int e = tid; e < KB * TB; e += 256) {
         int const kk = e / TB, x = e % TB;
@@ -461,134 +532,281 @@ __global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0
     }
 }
 
-// Forward solve L y = b, front by front from the leaves.  Slot vector w (m):
-// y of the pivots in [0, p) and the update passed to the parent in [p, m).
-template <typename T>
+// Forward solve L y = b (TR: U^T z = b), front by front from the leaves (large
+// fronts, one workgroup each, front in global memory).  Slot vector w (m): the
+// pivot results in [0, p) and the update passed to the parent in [p, m).
+// The transposed pair solves J^T lambda = g on the same factors: U^T is lower
+// triangular with U's diagonal, L^T unit upper; the right-hand side is then
+// indexed by solution slots and the result by right-hand-side slots.
+template <typename T, bool TR>
 __global__ void __launch_bounds__(256) k_front_fwd(FrontArgs a, int begin, const T* front, T* vec,
                                                    const double* rhs, double rhs_sign) {
-    int const s = a.level_front[begin + blockIdx.x];
-    int const inst = a.f_inst[s];
-    if (a.done && a.done[inst]) return;
-    int const m = a.f_m[s], p = a.f_p[s];
+    FrontMeta const fm = a.meta[begin + blockIdx.x];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
     int const tid = threadIdx.x, bs = blockDim.x;
-    const T* F = front + a.f_off[s];
-    T* w = vec + a.f_voff[s];
-    const int* rows = a.slot_row + a.f_slot_ptr[s];
+    const T* F = front + fm.off;
+    T* w = vec + fm.voff;
+    const int* rows = (TR ? a.slot_col : a.slot_row) + fm.slot_beg;
     for (int t = tid; t < m; t += bs) w[t] = t < p ? static_cast<T>(rhs_sign * rhs[rows[t]]) : T(0);
     __syncthreads();
-    for (int cq = a.f_child_ptr[s]; cq < a.f_child_ptr[s + 1]; ++cq) {
-        int const c = a.child[cq];
-        int const nu = a.f_m[c] - a.f_p[c];
-        const T* wc = vec + a.f_voff[c] + a.f_p[c];
-        const int* rl = a.rel + a.f_rel_off[c];
+    for (int cq = fm.child_beg; cq < fm.child_end; ++cq) {
+        ChildInfo const c = a.cinfo[cq];
+        int const nu = c.mc - c.pc;
+        const T* wc = vec + c.voff + c.pc;
+        const int* rl = a.rel + c.rel_off;
         for (int r = tid; r < nu; r += bs) w[rl[r]] += wc[r];
         __syncthreads();
     }
     for (int k = 0; k < p; ++k) {
-        T const xk = w[k];
-        for (int i = k + 1 + tid; i < m; i += bs) w[i] -= F[static_cast<long long>(i) * m + k] * xk;
+        T xk = w[k];
+        if constexpr (TR) xk = xk / F[static_cast<long long>(k) * m + k];
+        for (int i = k + 1 + tid; i < m; i += bs) {
+            T const c = TR ? F[static_cast<long long>(k) * m + i] : F[static_cast<long long>(i) * m + k];
+            w[i] -= c * xk;
+        }
+        __syncthreads();
+        if (TR && tid == 0) w[k] = xk;
         __syncthreads();
     }
 }
 
-// Backward solve U x = y, front by front from the roots; structure values come
-// from the (already solved) ancestors in the global solution vector.
-template <typename T>
+// Backward solve U x = y (TR: L^T x = z), front by front from the roots;
+// structure values come from the (already solved) ancestors in the global
+// solution vector.
+template <typename T, bool TR>
 __global__ void __launch_bounds__(256) k_front_bwd(FrontArgs a, int begin, const T* front, T* vec,
                                                    double* sol) {
-    int const s = a.level_front[begin + blockIdx.x];
-    int const inst = a.f_inst[s];
-    if (a.done && a.done[inst]) return;
-    int const m = a.f_m[s], p = a.f_p[s];
+    FrontMeta const fm = a.meta[begin + blockIdx.x];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
     int const tid = threadIdx.x, bs = blockDim.x;
-    const T* F = front + a.f_off[s];
-    T* w = vec + a.f_voff[s];
-    const int* cols = a.slot_col + a.f_slot_ptr[s];
+    const T* F = front + fm.off;
+    T* w = vec + fm.voff;
+    const int* cols = (TR ? a.slot_row : a.slot_col) + fm.slot_beg;
     for (int t = p + tid; t < m; t += bs) w[t] = static_cast<T>(sol[cols[t]]);
     __syncthreads();
     for (int k = tid; k < p; k += bs) {
         T acc = w[k];
-        const T* row = F + static_cast<long long>(k) * m;
-        for (int t = p; t < m; ++t) acc -= row[t] * w[t];
+        for (int t = p; t < m; ++t)
+            acc -= (TR ? F[static_cast<long long>(t) * m + k] : F[static_cast<long long>(k) * m + t]) * w[t];
         w[k] = acc;
     }
     __syncthreads();
-    // w[k] is only read in step k (every thread divides it itself), so the
+    // w[k] is only read in step k (every thread computes x_k itself), so the
     // result goes straight to the solution vector and w[k] stays untouched.
     for (int k = p - 1; k >= 0; --k) {
-        T const xk = w[k] / F[static_cast<long long>(k) * m + k];
-        for (int i = tid; i < k; i += bs) w[i] -= F[static_cast<long long>(i) * m + k] * xk;
+        T const xk = TR ? w[k] : w[k] / F[static_cast<long long>(k) * m + k];
+        for (int i = tid; i < k; i += bs)
+            w[i] -= (TR ? F[static_cast<long long>(k) * m + i] : F[static_cast<long long>(i) * m + k]) * xk;
         if (tid == 0) sol[cols[k]] = static_cast<double>(xk);
         __syncthreads();
     }
 }
 
-// Small-front forward / backward solves: one workgroup per front with the
-// front staged in LDS.
-template <typename T, int MM, int NT>
-__global__ void __launch_bounds__(NT) k_front_fwd_lds(FrontArgs a, int begin, const T* front, T* vec,
+// Small-front forward / backward solves, G fronts per workgroup as above
+// (TR: the transposed pair, see k_front_fwd).
+template <typename T, int MM, int L, int NT, bool TR>
+__global__ void __launch_bounds__(NT) k_front_fwd_lds(FrontArgs a, int begin, int count, const T* front, T* vec,
                                                       const double* rhs, double rhs_sign) {
-    constexpr int LD = MM + 1;
-    __shared__ T F[MM * LD];
-    __shared__ T w[MM];
-    int const s = a.level_front[begin + blockIdx.x];
-    int const inst = a.f_inst[s];
-    if (a.done && a.done[inst]) return;
-    int const m = a.f_m[s], p = a.f_p[s];
-    int const t = threadIdx.x;
-    const T* G = front + a.f_off[s];
-    for (int r = 0; r < m; ++r)
-        if (t < m) F[r * LD + t] = G[r * m + t];
-    const int* rows = a.slot_row + a.f_slot_ptr[s];
-    if (t < m) w[t] = t < p ? static_cast<T>(rhs_sign * rhs[rows[t]]) : T(0);
+    constexpr int G = NT / L, LD = MM + 1, CH = MM < 8 ? MM : 8;
+    __shared__ T Fs[G][MM * LD];
+    __shared__ T ws[G][MM];
+    int const g = threadIdx.x / L, t = threadIdx.x % L;
+    int const idx = blockIdx.x * G + g;
+    FrontMeta fm{};
+    if (idx < count) {
+        fm = a.meta[begin + idx];
+        if (a.done && a.done[fm.inst]) fm.m = fm.p = fm.child_end = fm.child_beg = 0;
+    }
+    int const m = fm.m, p = fm.p;
+    T* F = Fs[g];
+    T* w = ws[g];
+    load_rows<T, MM, L, CH>(F, LD, front + fm.off, m, TR ? p : m, t, t < m);
+    const int* rows = TR ? a.slot_col : a.slot_row;
+    if (t < m) w[t] = t < p ? static_cast<T>(rhs_sign * rhs[rows[fm.slot_beg + t]]) : T(0);
     __syncthreads();
-    for (int cq = a.f_child_ptr[s]; cq < a.f_child_ptr[s + 1]; ++cq) {
-        int const c = a.child[cq];
-        int const nu = a.f_m[c] - a.f_p[c];
-        const T* wc = vec + a.f_voff[c] + a.f_p[c];
-        const int* rl = a.rel + a.f_rel_off[c];
-        if (t < nu) w[rl[t]] += wc[t];
+    int const nch = fm.child_end - fm.child_beg;
+    int const nch_max = group_max<L, NT>(nch);
+    for (int ci = 0; ci < nch_max; ++ci) {
+        if (ci < nch) {
+            ChildInfo const c = a.cinfo[fm.child_beg + ci];
+            int const nu = c.mc - c.pc;
+            if (t < nu) w[a.rel[c.rel_off + t]] += vec[c.voff + c.pc + t];
+        }
         __syncthreads();
     }
-    for (int k = 0; k < p; ++k) {
-        T const xk = w[k];
-        if (t > k && t < m) w[t] -= F[t * LD + k] * xk;
+    // barriers are unconditional: fronts sharing a wavefront may have different p
+    int const p_max = group_max<L, NT>(p);
+    for (int k = 0; k < p_max; ++k) {
+        bool const act = k < p;
+        T xk = T(0);
+        if (act) {
+            xk = w[k];
+            if constexpr (TR) xk = xk / F[k * LD + k];
+            if (t > k && t < m) w[t] -= (TR ? F[k * LD + t] : F[t * LD + k]) * xk;
+        }
         __syncthreads();
+        // step k+1 reads only w[k+1..], so the owner may store x_k now
+        if (TR && act && t == k) w[k] = xk;
     }
-    T* wout = vec + a.f_voff[s];
-    if (t < m) wout[t] = w[t];
+    __syncthreads();
+    if (t < m) vec[fm.voff + t] = w[t];
 }
 
-template <typename T, int MM, int NT>
-__global__ void __launch_bounds__(NT) k_front_bwd_lds(FrontArgs a, int begin, const T* front, T* vec,
+template <typename T, int MM, int L, int NT, bool TR>
+__global__ void __launch_bounds__(NT) k_front_bwd_lds(FrontArgs a, int begin, int count, const T* front, T* vec,
                                                       double* sol) {
-    constexpr int LD = MM + 1;
-    __shared__ T F[MM * LD];
-    __shared__ T w[MM];
-    int const s = a.level_front[begin + blockIdx.x];
-    int const inst = a.f_inst[s];
-    if (a.done && a.done[inst]) return;
-    int const m = a.f_m[s], p = a.f_p[s];
-    int const t = threadIdx.x;
-    const T* G = front + a.f_off[s];
-    for (int r = 0; r < p; ++r)
-        if (t < m) F[r * LD + t] = G[r * m + t];
-    const int* cols = a.slot_col + a.f_slot_ptr[s];
-    const T* win = vec + a.f_voff[s];
-    if (t < m) w[t] = t < p ? win[t] : static_cast<T>(sol[cols[t]]);
+    constexpr int G = NT / L, LD = MM + 1, CH = MM < 8 ? MM : 8;
+    __shared__ T Fs[G][MM * LD];
+    __shared__ T ws[G][MM];
+    int const g = threadIdx.x / L, t = threadIdx.x % L;
+    int const idx = blockIdx.x * G + g;
+    FrontMeta fm{};
+    if (idx < count) {
+        fm = a.meta[begin + idx];
+        if (a.done && a.done[fm.inst]) fm.m = fm.p = 0;
+    }
+    int const m = fm.m, p = fm.p;
+    T* F = Fs[g];
+    T* w = ws[g];
+    const int* cols = (TR ? a.slot_row : a.slot_col) + fm.slot_beg;
+    load_rows<T, MM, L, CH>(F, LD, front + fm.off, m, TR ? m : p, t, t < m);
+    if (t < m) w[t] = t < p ? vec[fm.voff + t] : static_cast<T>(sol[cols[t]]);
     __syncthreads();
     if (t < p) {
         T acc = w[t];
-        for (int q = p; q < m; ++q) acc -= F[t * LD + q] * w[q];
+        for (int q = p; q < m; ++q) acc -= (TR ? F[q * LD + t] : F[t * LD + q]) * w[q];
         w[t] = acc;
     }
     __syncthreads();
-    for (int k = p - 1; k >= 0; --k) {
-        T const xk = w[k] / F[k * LD + k];
-        if (t < k) w[t] -= F[t * LD + k] * xk;
-        if (t == k) sol[cols[k]] = static_cast<double>(xk);
+    int const p_max = group_max<L, NT>(p);
+    for (int k = p_max - 1; k >= 0; --k) {
+        if (k < p) {
+            T const xk = TR ? w[k] : w[k] / F[k * LD + k];
+            if (t < k) w[t] -= (TR ? F[k * LD + t] : F[t * LD + k]) * xk;
+            if (t == k) sol[cols[k]] = static_cast<double>(xk);
+        }
         __syncthreads();
     }
+}
+
+// ------------------------------------------------------------ adjoint
+// dL/dx at the converged state, one thread per bus, for
+//   L = w_loss * sum_i Re(S_i) + w_dev * sum_PQ (|V_i| - 1)^2 + sum g_vm |V| + sum g_va arg V.
+// sum_i Re(S_i) = sum_ij G_ij Re(V_i conj V_j) (B drops out of the symmetric sum),
+// so its gradient in rectangular form is 2 G V; polar: d/dtheta_i =
+// 2 Im(conj(V_i) (G V)_i), d/d|V_i| = 2 Re(conj(V_i) (G V)_i) / |V_i|.
+struct AdjArgs {
+    int n_bus;
+    const int* rp;
+    const int* ci;
+    const double2* y;
+    const double2* v;
+    const double2* s;
+    const int* ang;
+    const int* mag;
+    const int* bus_type;
+    const double* g_vm;  // may be null
+    const double* g_va;  // may be null
+    double w_loss, w_dev;
+    double* rhs;     // per Newton variable
+    double* lossb;   // per bus loss contribution
+};
+__global__ void __launch_bounds__(256) k_adj_rhs(AdjArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_bus) return;
+    double ge = 0.0, gf = 0.0;
+    for (int q = a.rp[i]; q < a.rp[i + 1]; ++q) {
+        double const G = a.y[q].x;
+        double2 const vj = a.v[a.ci[q]];
+        ge += G * vj.x;
+        gf += G * vj.y;
+    }
+    double2 const vi = a.v[i];
+    double const vm = hypot(vi.x, vi.y);
+    double const dth = 2.0 * a.w_loss * (vi.x * gf - vi.y * ge) + (a.g_va ? a.g_va[i] : 0.0);
+    double dvm = 2.0 * a.w_loss * (vi.x * ge + vi.y * gf) / vm + (a.g_vm ? a.g_vm[i] : 0.0);
+    bool const pq = a.bus_type[i] == 0;
+    if (pq) dvm += 2.0 * a.w_dev * (vm - 1.0);
+    if (a.ang[i] >= 0) a.rhs[a.ang[i]] = dth;
+    if (a.mag[i] >= 0) a.rhs[a.mag[i]] = dvm;
+    double l = a.w_loss * a.s[i].x + (a.g_vm ? a.g_vm[i] * vm : 0.0) + (a.g_va ? a.g_va[i] * atan2(vi.y, vi.x) : 0.0);
+    if (pq) l += a.w_dev * (vm - 1.0) * (vm - 1.0);
+    a.lossb[i] = l;
+}
+
+// dL/d(bus p, q): F_P = P - p/S_base, so dL/dp = lambda_P / S_base (same for Q).
+struct AdjBusArgs {
+    int n_bus;
+    const int* peq;
+    const int* qeq;
+    const int* bus_inst;
+    const double* lam;
+    double inv_base;
+    double* gp;
+    double* gq;
+};
+__global__ void __launch_bounds__(256) k_adj_bus(AdjBusArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_bus) return;
+    a.gp[i] = a.peq[i] >= 0 ? a.lam[a.peq[i]] * a.inv_base : 0.0;
+    a.gq[i] = a.qeq[i] >= 0 ? a.lam[a.qeq[i]] * a.inv_base : 0.0;
+}
+
+// dL/d(branch r, x) = w_loss * Re(dy) |V_f - V_t|^2 - lambda^T dF/d(r, x), with
+// dy/dr = -Zb y_S^2 and dy/dx = -j Zb y_S^2 (y_S = 1 / (r + jx) in siemens),
+// dS_f = V_f conj(dy (V_f - V_t)), dS_t = V_t conj(dy (V_t - V_f)).
+struct AdjBrArgs {
+    int n_br;
+    const int* bf;
+    const int* bt;
+    const int* br_inst;
+    const double* r;
+    const double* x;
+    const double* zb;
+    const double2* v;
+    const int* peq;
+    const int* qeq;
+    const double* lam;
+    double w_loss;
+    double* gr;
+    double* gx;
+};
+__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+__global__ void __launch_bounds__(256) k_adj_branch(AdjBrArgs a) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n_br) return;
+    int const f = a.bf[k], t = a.bt[k];
+    double gr = 0.0, gx = 0.0;
+    double const r = a.r[k], x = a.x[k];
+    double const z2 = r * r + x * x;
+    if (f >= 0 && z2 > 1e-12) {
+        double2 const y = make_double2(r / z2, -x / z2);
+        double2 y2 = cmul(y, y);
+        double const zb = a.zb[a.br_inst[k]];
+        double2 const dyr = make_double2(-zb * y2.x, -zb * y2.y);
+        double2 const dyx = make_double2(zb * y2.y, -zb * y2.x);  // -j zb y^2
+        double2 const vf = a.v[f], vt = a.v[t];
+        double2 const dv = make_double2(vf.x - vt.x, vf.y - vt.y);
+        double const dv2 = dv.x * dv.x + dv.y * dv.y;
+        for (int which = 0; which < 2; ++which) {
+            double2 const dy = which == 0 ? dyr : dyx;
+            double2 const dsf = cmul(vf, cconj(cmul(dy, dv)));
+            double2 const dst = cmul(vt, cconj(cmul(dy, make_double2(-dv.x, -dv.y))));
+            double lt = 0.0;
+            if (a.peq[f] >= 0) lt += a.lam[a.peq[f]] * dsf.x;
+            if (a.qeq[f] >= 0) lt += a.lam[a.qeq[f]] * dsf.y;
+            if (a.peq[t] >= 0) lt += a.lam[a.peq[t]] * dst.x;
+            if (a.qeq[t] >= 0) lt += a.lam[a.qeq[t]] * dst.y;
+            double const g = a.w_loss * dy.x * dv2 - lt;
+            if (which == 0) gr = g; else gx = g;
+        }
+    }
+    a.gr[k] = gr;
+    a.gx[k] = gx;
 }
 
 // ------------------------------------------------------------ update

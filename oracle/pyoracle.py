@@ -170,6 +170,28 @@ class Oracle(_Solver):
 
     def __init__(self):
         super().__init__(ORACLE_SO)
+        self._jac = self.lib.pfo_jacobian
+        self._jac.restype = C.c_int
+
+    def jacobian(self, net, v, ypu):
+        """Jacobian (reference layout, |J| > 1e-12) of the per-unit Y at voltages v."""
+        s, keep = make_pfnet(net)
+        rp, ci, vals = ypu
+        vri = np.ascontiguousarray(np.stack([np.real(v), np.imag(v)], 1).ravel())
+        yri = np.ascontiguousarray(np.stack([np.real(vals), np.imag(vals)], 1).ravel())
+        rp = np.ascontiguousarray(rp, np.int32)
+        ci = np.ascontiguousarray(ci, np.int32)
+        n = len(net.bus_id)
+        cap = 8 * (len(ci) + n) + 16
+        jn = C.c_int(0)
+        jrp = np.zeros(2 * n + 2, np.int32)
+        jci = np.zeros(cap, np.int32)
+        jv = np.zeros(cap)
+        nnz = self._jac(C.byref(s), _ptr(vri, _d), _ptr(yri, _d), _ptr(rp, _i), _ptr(ci, _i), C.byref(jn),
+                        _ptr(jrp, _i), _ptr(jci, _i), _ptr(jv, _d), cap)
+        if nnz < 0:
+            raise RuntimeError("Jacobian capacity")
+        return jrp[:jn.value + 1].copy(), jci[:nnz].copy(), jv[:nnz].copy()
 
 
 def reference_available() -> bool:
