@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=1536)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="after the run, copy a 512 MiB device tensor 3 times (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batch shards by "
@@ -162,6 +164,14 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.probe_copy:
+        n = 512 * 1024 * 1024 // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 3 x {4 * n} bytes read and written")
 
 
 def cpu_baseline(nets, sizes, count, tol, max_it, base):
