@@ -98,6 +98,20 @@ def main():
     log(f"[bench] batched mixed: {value:.0f} solves/s ({ms_per_step:.2f} ms/step), iterations p50 "
         f"{np.percentile(its, 50):.0f} p99 {np.percentile(its, 99):.0f}")
 
+    # the same plan on the level-synchronous schedule (one launch per phase and tree level)
+    levels_value = None
+    if plan.schedule == "grid":
+        plan.set_schedule("levels")
+        plan.solve(tol, max_it, base, fetch_v=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(1, args.steps // 2)):
+            plan.solve(tol, max_it, base, fetch_v=False)
+        torch.cuda.synchronize()
+        levels_value = len(nets) * max(1, args.steps // 2) / (time.perf_counter() - t0)
+        plan.set_schedule("auto")
+        log(f"[bench] batched mixed, level schedule: {levels_value:.0f} solves/s")
+
     # accuracy of the mixed path against the FP64 path on the same batch
     r_mixed = plan.solve(tol, max_it, base, fetch_v=True)
     plan64 = capi.Plan(nets, "fp64")
@@ -111,20 +125,26 @@ def main():
     batched_fp64 = len(nets) * max(1, args.steps // 2) / (time.perf_counter() - t0)
     del plan64, r64, r_mixed
 
-    # plain FP64 single-instance GPU path on a stratified sample of the same grids
+    # plain FP64 single-instance GPU path on a stratified sample of the same grids,
+    # on both schedules (level-synchronous, and one workgroup for the grid)
     idx = stratified(sizes, min(args.plain_sample, len(nets)))
     plans = [capi.Plan([nets[i]], "fp64") for i in idx]
-    for p in plans[:4]:
-        p.solve(tol, max_it, base, fetch_v=False)
-    times = []
-    for p in plans:
-        t1 = time.perf_counter()
-        p.solve(tol, max_it, base, fetch_v=False)
-        times.append(time.perf_counter() - t1)
-    plain = len(times) / sum(times)
+    single = {}
+    for sched in ("levels", "auto"):
+        for p in plans:
+            p.set_schedule(sched)
+        for p in plans[:4]:
+            p.solve(tol, max_it, base, fetch_v=False)
+        times = []
+        for p in plans:
+            t1 = time.perf_counter()
+            p.solve(tol, max_it, base, fetch_v=False)
+            times.append(time.perf_counter() - t1)
+        single[sched] = len(times) / sum(times)
     del plans
-    log(f"[bench] plain FP64 single-instance: {plain:.1f} solves/s (sample of {len(times)}); "
-        f"batched FP64 {batched_fp64:.0f} solves/s")
+    plain = max(single.values())
+    log(f"[bench] plain FP64 single-instance: levels {single['levels']:.1f}, auto {single['auto']:.1f} solves/s "
+        f"(sample of {len(idx)}); batched FP64 {batched_fp64:.0f} solves/s")
 
     # live per-kernel timing (HIP events around every launch on the plan's stream)
     plan.profile(True)
@@ -155,8 +175,11 @@ def main():
                                "edges), loads x U[0.7,1.3] x 0.5, one plan, mixed precision",
                    "grids": len(nets), "buses": int(sizes.sum()), "tolerance_pu": tol,
                    "base_power_va": base, "max_iterations": max_it},
+        "schedule": plan.schedule,
         "plain_fp64_solves_per_s": round(plain, 2),
+        "plain_fp64_by_schedule": {k: round(v, 2) for k, v in single.items()},
         "speedup_vs_plain_fp64": round(value / plain, 2),
+        "batched_mixed_level_schedule_solves_per_s": None if levels_value is None else round(levels_value, 1),
         "batched_fp64_solves_per_s": round(batched_fp64, 1),
         "iterations_p50": float(np.percentile(its, 50)), "iterations_p99": float(np.percentile(its, 99)),
         "mixed_vs_fp64_max_abs_dv": dv, "mixed_vs_fp64_max_iteration_diff": it_diff,

@@ -20,6 +20,8 @@ PLUGINS_SO = LIB_DIR / "libgap_hip_plugins.so"
 
 OK, E_ARG, E_DEVICE, E_CAPACITY, E_SINGULAR, E_STATE = 0, -1, -2, -3, -4, -5
 FP64, MIXED = 0, 1
+SCHED_AUTO, SCHED_LEVELS, SCHED_GRID = 0, 1, 2
+_SCHED = {"auto": SCHED_AUTO, "levels": SCHED_LEVELS, "grid": SCHED_GRID}
 
 _i = C.POINTER(C.c_int)
 _d = C.POINTER(C.c_double)
@@ -48,10 +50,13 @@ class PlanStats(C.Structure):
                 ("ybus_nnz", C.c_longlong), ("n_var", C.c_longlong), ("jac_nnz", C.c_longlong),
                 ("n_fronts", C.c_longlong), ("n_levels", C.c_longlong),
                 ("front_entries", C.c_longlong), ("factor_entries", C.c_longlong),
-                ("max_front", C.c_longlong), ("analysis_ms", C.c_double)]
+                ("max_front", C.c_longlong), ("analysis_ms", C.c_double),
+                ("front_hist", C.c_longlong * 8)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["front_hist"] = list(self.front_hist)
+        return d
 
 
 class SolveOpts(C.Structure):
@@ -71,6 +76,7 @@ SIGNATURES = {
     "gaphip_plan_set_injections": (C.c_int, [C.c_void_p, C.c_int, _d, _d]),
     "gaphip_plan_set_branch_rx": (C.c_int, [C.c_void_p, C.c_int, _d, _d]),
     "gaphip_plan_set_ybus": (C.c_int, [C.c_void_p, C.c_int, _i, _i, _d, C.c_int, C.c_int]),
+    "gaphip_plan_set_schedule": (C.c_int, [C.c_void_p, C.c_int, _i]),
     "gaphip_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), _d, _i, _i, _d]),
     "gaphip_trajectory": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_int, _i]),
     "gaphip_jacobian": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_double, _i, _i, _i, _d, C.c_int, _i]),
@@ -170,7 +176,7 @@ def ybus(net: Network):
 class Plan:
     """A batch of networks solved together on the GPU (one plan, packed)."""
 
-    def __init__(self, nets: Sequence[Network], precision: int | str = FP64):
+    def __init__(self, nets: Sequence[Network], precision: int | str = FP64, schedule: int | str = SCHED_AUTO):
         if isinstance(precision, str):
             precision = {"fp64": FP64, "mixed": MIXED}[precision]
         self.nets = list(nets)
@@ -186,6 +192,16 @@ class Plan:
         self._h = h
         self._keep = None
         self.bus_off = np.concatenate([[0], np.cumsum([n.n_bus for n in self.nets])]).astype(np.int64)
+        self.schedule = self.set_schedule(schedule)
+
+    def set_schedule(self, schedule: int | str = SCHED_AUTO) -> str:
+        """Choose the device schedule ('auto', 'levels', 'grid'); returns the one solves will use."""
+        if isinstance(schedule, str):
+            schedule = _SCHED[schedule]
+        eff = C.c_int(0)
+        _check(lib().gaphip_plan_set_schedule(self._h, int(schedule), C.byref(eff)), "set_schedule")
+        self.schedule = {SCHED_LEVELS: "levels", SCHED_GRID: "grid"}[eff.value]
+        return self.schedule
 
     def close(self):
         if getattr(self, "_h", None):

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -19,6 +20,7 @@
 
 #include "../../include/gap_hip.h"
 #include "kernels.hpp"
+#include "fused.hpp"
 #include "symbolic.hpp"
 
 namespace gaphip {
@@ -65,16 +67,16 @@ struct DBuf {
 inline int blocks_for(long long n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
 
 // Kernel ids of the timing hook: one per kernel (front kernels per size class).
-enum Kid { K_YBUS = 0, K_FLAT, K_MISMATCH, K_CHECK, K_JAC, K_UPDATE, K_FACTOR, K_FWD = K_FACTOR + 6, K_BWD = K_FWD + 6,
-           K_COUNT = K_BWD + 6 };
+enum Kid { K_YBUS = 0, K_FLAT, K_MISMATCH, K_CHECK, K_JAC, K_UPDATE, K_FACTOR, K_FWD = K_FACTOR + 7, K_BWD = K_FWD + 7,
+           K_FUSED = K_BWD + 7, K_COUNT };
 static char const* const kKernelNames[K_COUNT] = {
     "k_ybus", "k_flat", "k_mismatch", "k_check", "k_jacobian", "k_update",
-    "k_front_factor_lds<8>", "k_front_factor_lds<16>", "k_front_factor_lds<32>", "k_front_factor_lds<64>",
-    "k_front_factor_lds<128>", "k_big_factor (assemble+panel+trsm+gemm)",
-    "k_front_fwd_lds<8>", "k_front_fwd_lds<16>", "k_front_fwd_lds<32>", "k_front_fwd_lds<64>",
-    "k_front_fwd_lds<128>", "k_front_fwd",
-    "k_front_bwd_lds<8>", "k_front_bwd_lds<16>", "k_front_bwd_lds<32>", "k_front_bwd_lds<64>",
-    "k_front_bwd_lds<128>", "k_front_bwd"};
+    "k_front_factor_lds<4>", "k_front_factor_lds<8>", "k_front_factor_lds<16>", "k_front_factor_lds<32>",
+    "k_front_factor_lds<64>", "k_front_factor_lds<128>", "k_big_factor (assemble+panel+trsm+gemm)",
+    "k_front_fwd_lds<4>", "k_front_fwd_lds<8>", "k_front_fwd_lds<16>", "k_front_fwd_lds<32>",
+    "k_front_fwd_lds<64>", "k_front_fwd_lds<128>", "k_front_fwd",
+    "k_front_bwd_lds<4>", "k_front_bwd_lds<8>", "k_front_bwd_lds<16>", "k_front_bwd_lds<32>",
+    "k_front_bwd_lds<64>", "k_front_bwd_lds<128>", "k_front_bwd", "k_nr_grid"};
 
 Net copy_net(gaphip_net const& in) {
     if (in.n_bus <= 0) throw Error(GAPHIP_E_ARG, "network has no buses");
@@ -180,16 +182,24 @@ LUInput nr_lu_input(Net const& net, YPlan const& y, NRPlan const& p, std::vector
     return in;
 }
 
-// Front size classes: LDS-resident kernels for fronts of up to 8 / 16 / 32 /
-// 64 / 128 slots (8, 4, 2, 1, 1 fronts per workgroup), the blocked
+// Front size classes: LDS-resident kernels for fronts of up to 4 / 8 / 16 /
+// 32 / 64 / 128 slots (16, 8, 4, 2, 1, 1 fronts per workgroup), the blocked
 // global-memory path above that.
-constexpr int kNumClasses = 6;
-constexpr int kClassMax[kNumClasses - 1] = {8, 16, 32, 64, 128};
-constexpr int kClassPerBlock[kNumClasses - 1] = {8, 4, 2, 1, 1};  // fronts per workgroup
+constexpr int kNumClasses = 7;
+constexpr int kClassMax[kNumClasses - 1] = {4, 8, 16, 32, 64, 128};
+constexpr int kClassPerBlock[kNumClasses - 1] = {16, 8, 4, 2, 1, 1};  // fronts per workgroup
+constexpr int kGridClasses = 5;  // classes the grid schedule handles (fronts <= 64 slots)
 inline int front_class(int m) {
     for (int c = 0; c < kNumClasses - 1; ++c)
         if (m <= kClassMax[c]) return c;
     return kNumClasses - 1;
+}
+
+// bins of gaphip_plan_stats::front_hist: <= 2, 4, 8, ..., 128 slots, above
+inline int front_hist_bin(int m) {
+    int b = 0;
+    for (int cap = 2; b < 7 && m > cap; cap <<= 1) ++b;
+    return b;
 }
 
 // Device copy of a (possibly batched) multifrontal plan.  Fronts are numbered
@@ -207,6 +217,19 @@ struct DevFronts {
     DBuf<int> bigbase, arow, crow, crow_ci;  // row maps of fronts above 128 slots
     long long front_entries = 0, vec_entries = 0, factor_entries = 0;
     int max_front = 0;
+    // per instance: arrowhead entries, factor entries (p*m + (m-p)*p summed), slots
+    std::vector<long long> inst_arrow, inst_factor, inst_slots;
+    // grid schedule (every front <= 64 slots): per instance its levels
+    // [fz_lvl_beg[g], fz_lvl_beg[g+1]) of fz_lvl_ptr, each a range of fz_tasks,
+    // and the flat assembly lists the tasks index
+    bool fz_ok = false;
+    int fz_pool = 0, fz_ld4 = 1;
+    DBuf<int> fz_order, fz_lvl_beg, fz_lvl_ptr;
+    DBuf<GridTask> fz_tasks;
+    DBuf<int4> fz_pairs;
+    DBuf<int2> fz_rhs, fz_ext, fz_vx, fz_sol;
+    long long fz_list_bytes = 0;
+    std::vector<double> fz_fixed, fz_welems;  // per instance, per Newton iteration (see build_grid)
 
     BigMaps big() const { return BigMaps{bigbase.p, arow.p, crow.p, crow_ci.p}; }
 
@@ -225,6 +248,211 @@ struct DevFronts {
     }
 };
 
+// Jacobian fill items of a plan (global bus / Y / J indices), the input of the
+// grid schedule's pair lists.
+struct GridItems {
+    std::vector<int> const& it_i;
+    std::vector<int> const& it_j;
+    std::vector<int> const& it_y;
+    std::vector<int> const& it_pos;
+};
+
+// Grid schedule of a packed plan: tasks (runs of at most 64 / L fronts of one
+// instance, level and class) and the flat assembly lists each task indexes.
+// Within a (level, class) bucket the fronts of one instance are contiguous
+// (the renumbering is stable), so a task is a contiguous id range.  Every
+// index and LDS position is checked here.
+void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems const& it,
+                std::vector<FrontMeta> const& meta, std::vector<ChildInfo> const& cinfo, std::vector<int> const& rel,
+                std::vector<int> const& arrow_src, std::vector<int> const& arrow_dst,
+                std::vector<int> const& slot_row, std::vector<int> const& slot_col, long long n_index,
+                long long n_values, hipStream_t s) {
+    int const G = static_cast<int>(plans.size());
+    int const nf = d.n_fronts, K = kNumClasses;
+    constexpr long long kIntMax = 0x7fffffffLL;
+    d.fz_ok = nf > 0 && d.max_front <= kClassMax[kGridClasses - 1] && d.front_entries < kIntMax &&
+              d.vec_entries < kIntMax && n_index < kIntMax && n_values < kIntMax;
+    if (!d.fz_ok) return;
+    // Jacobian entry -> (front, position) from the arrowhead lists
+    std::vector<int> jfront(n_values, -1), jdst(n_values, 0);
+    for (int n = 0; n < nf; ++n)
+        for (int q = meta[n].arrow_beg; q < meta[n].arrow_end; ++q) {
+            jfront[arrow_src[q]] = n;
+            jdst[arrow_src[q]] = arrow_dst[q];
+        }
+    // one pair per Jacobian 2x2 block: its front, corner (P row, theta column) and entry mask
+    long long const n_items = static_cast<long long>(it.it_i.size());
+    std::vector<int> pf(n_items, -1), pcorner(n_items, 0), pmask(n_items, 0), fpair_ptr(nf + 1, 0);
+    for (long long t = 0; t < n_items; ++t) {
+        int f = -1, r0 = 0, c0 = 0, mask = 0;
+        for (int q = 0; q < 4; ++q) {
+            int const pos = it.it_pos[4 * t + q];
+            if (pos < 0) continue;
+            int const fr = jfront[pos];
+            if (fr < 0) throw Error(GAPHIP_E_ARG, "Jacobian entry without a front");
+            int const r = (jdst[pos] >> 16) - (q >> 1), c = (jdst[pos] & 0xffff) - (q & 1);
+            if (f < 0) {
+                f = fr, r0 = r, c0 = c;
+            } else if (fr != f || r != r0 || c != c0) {
+                d.fz_ok = false;  // 2x2 block split over fronts or slots: not expressible as one pair
+                return;
+            }
+            mask |= 1 << q;
+        }
+        if (f < 0) continue;
+        if (r0 < 0 || c0 < 0) throw Error(GAPHIP_E_ARG, "Jacobian block corner outside its front");
+        pf[t] = f;
+        pcorner[t] = (r0 << 16) | c0;
+        pmask[t] = mask;
+        fpair_ptr[f + 1]++;
+    }
+    for (int n = 0; n < nf; ++n) fpair_ptr[n + 1] += fpair_ptr[n];
+    std::vector<int> fpair(fpair_ptr[nf]);
+    {
+        std::vector<int> fill(fpair_ptr.begin(), fpair_ptr.end() - 1);
+        for (long long t = 0; t < n_items; ++t)
+            if (pf[t] >= 0) fpair[fill[pf[t]]++] = static_cast<int>(t);
+    }
+    // tasks per (instance, level), classes in increasing order
+    std::vector<int> lvl_beg(G + 1, 0);
+    for (int g = 0; g < G; ++g) lvl_beg[g + 1] = lvl_beg[g] + plans[g]->n_levels;
+    std::vector<std::vector<int2>> per(lvl_beg[G]);
+    int m4 = 0;
+    for (int l = 0; l < d.n_levels; ++l)
+        for (int c = 0; c < kGridClasses; ++c) {
+            size_t const b = static_cast<size_t>(l) * K + c;
+            int const per_task = kWave / kClassMax[c];
+            for (int n = d.cls_ptr[b]; n < d.cls_ptr[b + 1];) {
+                int const g = meta[n].inst;
+                int e = n;
+                while (e < d.cls_ptr[b + 1] && meta[e].inst == g && e - n < per_task) ++e;
+                if (l >= plans[g]->n_levels) throw Error(GAPHIP_E_ARG, "front level outside its network's tree");
+                per[lvl_beg[g] + l].push_back(make_int2(n, (c << 8) | (e - n)));
+                if (c == kGridClasses - 1)
+                    for (int q = n; q < e; ++q) m4 = std::max(m4, meta[q].m);
+                n = e;
+            }
+        }
+    d.fz_ld4 = m4 + 1;
+    std::vector<int> lvl_ptr(1, 0);
+    std::vector<GridTask> tasks;
+    std::vector<int4> pairs;
+    std::vector<int2> rhs, ext, vx, sol;
+    int pool = 1;
+    for (auto const& lv : per) {
+        for (int2 const& tr : lv) {
+            int const c = tr.y >> 8, cnt = tr.y & 0xff, L = kClassMax[c], Gc = kWave / L;
+            int const ld = c == kGridClasses - 1 ? d.fz_ld4 : L + 1, rows = ld - 1;
+            int const wbase = Gc * rows * ld, region = wbase + Gc * L;
+            pool = std::max(pool, region);
+            GridTask tk{};
+            tk.front_beg = tr.x;
+            tk.cls_cnt = tr.y;
+            tk.pair_beg = static_cast<int>(pairs.size());
+            tk.rhs_beg = static_cast<int>(rhs.size());
+            tk.ext_beg = static_cast<int>(ext.size());
+            tk.vx_beg = static_cast<int>(vx.size());
+            tk.sol_beg = static_cast<int>(sol.size());
+            for (int gi = 0; gi < cnt; ++gi) {
+                int const n = tr.x + gi;
+                FrontMeta const& fm = meta[n];
+                if (front_class(fm.m) != c || fm.m > rows) throw Error(GAPHIP_E_ARG, "grid task class mismatch");
+                int const tile = gi * rows * ld, wpos = wbase + gi * L;
+                auto in_tile = [&](int r, int col) { return r >= 0 && col >= 0 && r < fm.m && col < fm.m; };
+                for (int q = fpair_ptr[n]; q < fpair_ptr[n + 1]; ++q) {
+                    int const t = fpair[q];
+                    int const r0 = pcorner[t] >> 16, c0 = pcorner[t] & 0xffff, mask = pmask[t];
+                    for (int e = 0; e < 4; ++e)
+                        if ((mask >> e & 1) && !in_tile(r0 + (e >> 1), c0 + (e & 1)))
+                            throw Error(GAPHIP_E_ARG, "Jacobian pair outside its front");
+                    pairs.push_back(make_int4(it.it_i[t], it.it_j[t], it.it_y[t], (tile + r0 * ld + c0) | (mask << 16)));
+                }
+                for (int t = 0; t < fm.p; ++t) rhs.push_back(make_int2(slot_row[fm.slot_beg + t], wpos + t));
+                for (int t = fm.p; t < fm.m; ++t) sol.push_back(make_int2(slot_col[fm.slot_beg + t], wpos + t));
+                for (int ci = fm.child_beg; ci < fm.child_end; ++ci) {
+                    ChildInfo const& ch = cinfo[ci];
+                    int const nu = ch.mc - ch.pc;
+                    int const* rl = rel.data() + ch.rel_off;
+                    for (int r = 0; r < nu; ++r) {
+                        if (rl[r] < 0 || rl[r] >= fm.m) throw Error(GAPHIP_E_ARG, "extend-add map out of range");
+                        for (int c2 = 0; c2 < nu; ++c2)
+                            ext.push_back(make_int2(static_cast<int>(ch.off + static_cast<long long>(ch.pc + r) * ch.mc +
+                                                                     ch.pc + c2),
+                                                    tile + rl[r] * ld + rl[c2]));
+                        vx.push_back(make_int2(static_cast<int>(ch.voff + ch.pc + r), wpos + rl[r]));
+                    }
+                }
+            }
+            tk.pair_end = static_cast<int>(pairs.size());
+            tk.rhs_end = static_cast<int>(rhs.size());
+            tk.ext_end = static_cast<int>(ext.size());
+            tk.vx_end = static_cast<int>(vx.size());
+            tk.sol_end = static_cast<int>(sol.size());
+            if (static_cast<long long>(ext.size()) >= kIntMax) throw Error(GAPHIP_E_ARG, "grid schedule lists too long");
+            tasks.push_back(tk);
+        }
+        lvl_ptr.push_back(static_cast<int>(tasks.size()));
+    }
+    // algorithmic bytes per Newton iteration of each instance (factor + backward):
+    // fz_fixed[g] bytes of indices and FP64 inputs, fz_welems[g] entries of the
+    // factor precision
+    d.fz_fixed.assign(G, 0.0);
+    d.fz_welems.assign(G, 0.0);
+    for (size_t k = 0; k < tasks.size(); ++k) {
+        GridTask const& tk = tasks[k];
+        int const g = meta[tk.front_beg].inst;
+        d.fz_fixed[g] += sizeof(GridTask) * 2.0 + 64.0 * (tk.pair_end - tk.pair_beg) + 16.0 * (tk.rhs_end - tk.rhs_beg) +
+                         8.0 * (tk.ext_end - tk.ext_beg) + 8.0 * (tk.vx_end - tk.vx_beg) + 16.0 * (tk.sol_end - tk.sol_beg);
+        d.fz_welems[g] += (tk.ext_end - tk.ext_beg) + (tk.vx_end - tk.vx_beg);
+        for (int n = tk.front_beg; n < tk.front_beg + (tk.cls_cnt & 0xff); ++n) {
+            double const m = meta[n].m, pp = meta[n].p;
+            d.fz_fixed[g] += 2.0 * sizeof(FrontMeta) + 8.0 * pp;  // metadata twice, solution writes
+            // stored U rows and Schur complement, w; then U rows and w read back
+            d.fz_welems[g] += (pp * m - pp * (pp - 1) / 2) + (m - pp) * (m - pp) + m + pp * m + pp;
+        }
+    }
+    // bounds of every gathered index and LDS position
+    for (auto const& pr : pairs)
+        if (pr.x < 0 || pr.y < 0 || pr.z < 0 || (pr.w & 0xffff) + d.fz_ld4 + 1 >= (1 << 16))
+            throw Error(GAPHIP_E_ARG, "pair out of range");
+    auto check2 = [&](std::vector<int2> const& v, long long src_max, char const* what) {
+        for (auto const& e : v)
+            if (e.x < 0 || e.x >= src_max || e.y < 0 || e.y >= pool) throw Error(GAPHIP_E_ARG, std::string(what) + " out of range");
+    };
+    check2(rhs, n_index, "rhs entry");
+    check2(sol, n_index, "solution entry");
+    check2(ext, d.front_entries, "extend-add entry");
+    check2(vx, d.vec_entries, "vector entry");
+    d.fz_pool = pool;
+    // largest networks first: they bound the launch
+    std::vector<int> order(G);
+    for (int g = 0; g < G; ++g) order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return d.inst_factor[x] + d.inst_slots[x] > d.inst_factor[y] + d.inst_slots[y];
+    });
+    auto nonempty = [](auto& v, auto z) {
+        if (v.empty()) v.push_back(z);
+    };
+    nonempty(tasks, GridTask{});
+    nonempty(pairs, make_int4(0, 0, 0, 0));
+    nonempty(rhs, make_int2(0, 0));
+    nonempty(ext, make_int2(0, 0));
+    nonempty(vx, make_int2(0, 0));
+    nonempty(sol, make_int2(0, 0));
+    d.fz_list_bytes = static_cast<long long>(tasks.size()) * sizeof(GridTask) + pairs.size() * sizeof(int4) +
+                      (rhs.size() + ext.size() + vx.size() + sol.size()) * sizeof(int2);
+    d.fz_order.upload(order, s);
+    d.fz_lvl_beg.upload(lvl_beg, s);
+    d.fz_lvl_ptr.upload(lvl_ptr, s);
+    d.fz_tasks.upload(tasks, s);
+    d.fz_pairs.upload(pairs, s);
+    d.fz_rhs.upload(rhs, s);
+    d.fz_ext.upload(ext, s);
+    d.fz_vx.upload(vx, s);
+    d.fz_sol.upload(sol, s);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
 // Concatenate per-instance LU plans and renumber the fronts by (level, class).
 // key_row / key_col map a slot key of instance g to its global right-hand-side /
 // solution index; ent_base[g] is the offset of instance g's entries in the
@@ -232,7 +460,7 @@ struct DevFronts {
 template <typename KeyRow, typename KeyCol>
 void pack_fronts(std::vector<LUPlan const*> const& plans, std::vector<long long> const& ent_base,
                  KeyRow key_row, KeyCol key_col, long long n_index, long long n_values, DevFronts& d,
-                 hipStream_t s) {
+                 hipStream_t s, GridItems const* items = nullptr) {
     int const G = static_cast<int>(plans.size());
     // old (instance-concatenated) numbering
     std::vector<int> o_inst, o_local, o_base(G + 1, 0);
@@ -407,6 +635,18 @@ void pack_fronts(std::vector<LUPlan const*> const& plans, std::vector<long long>
     if (crow_ci.empty()) crow_ci.assign(2, 0);
     if (cinfo.empty()) cinfo.push_back(ChildInfo{});
     if (rel.empty()) rel.push_back(0);
+    // per-instance totals (algorithmic byte counts of the grid-resident kernel)
+    d.inst_arrow.assign(G, 0);
+    d.inst_factor.assign(G, 0);
+    d.inst_slots.assign(G, 0);
+    for (int n = 0; n < nf; ++n) {
+        long long m = meta[n].m, pp = meta[n].p;
+        d.inst_arrow[meta[n].inst] += meta[n].arrow_end - meta[n].arrow_beg;
+        d.inst_factor[meta[n].inst] += pp * m + (m - pp) * pp;
+        d.inst_slots[meta[n].inst] += m;
+    }
+    if (items) build_grid(d, plans, *items, meta, cinfo, rel, arrow_src, arrow_dst, slot_row, slot_col, n_index,
+                          n_values, s);
     d.meta.upload(meta, s);
     d.cinfo.upload(cinfo, s);
     d.rel.upload(rel, s);
@@ -453,11 +693,12 @@ struct FrontRunner {
                 hook(kid + c, bytes, [&] {
                     int const gb = c < kNumClasses - 1 ? blocks_for(nb, kClassPerBlock[c]) : 0;
                     switch (c) {
-                        case 0: hipLaunchKernelGGL((k_front_factor_lds<T, 8, 8, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
-                        case 1: hipLaunchKernelGGL((k_front_factor_lds<T, 16, 16, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
-                        case 2: hipLaunchKernelGGL((k_front_factor_lds<T, 32, 32, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
-                        case 3: hipLaunchKernelGGL((k_front_factor_lds<T, 64, 64, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
-                        case 4: hipLaunchKernelGGL((k_front_factor_lds<T, 128, 128, 128>), dim3(gb), dim3(128), 0, s, a, beg, nb, jv, front); break;
+                        case 0: hipLaunchKernelGGL((k_front_factor_lds<T, 4, 4, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
+                        case 1: hipLaunchKernelGGL((k_front_factor_lds<T, 8, 8, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
+                        case 2: hipLaunchKernelGGL((k_front_factor_lds<T, 16, 16, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
+                        case 3: hipLaunchKernelGGL((k_front_factor_lds<T, 32, 32, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
+                        case 4: hipLaunchKernelGGL((k_front_factor_lds<T, 64, 64, 64>), dim3(gb), dim3(64), 0, s, a, beg, nb, jv, front); break;
+                        case 5: hipLaunchKernelGGL((k_front_factor_lds<T, 128, 128, 128>), dim3(gb), dim3(128), 0, s, a, beg, nb, jv, front); break;
                         default: big_factor(d, a, beg, nb, d.cls_mmax[b], d.cls_pmax[b], jv, front, s);
                     }
                 });
@@ -476,11 +717,12 @@ struct FrontRunner {
                 hook(kid + c, bytes, [&] {
                     int const gb = c < kNumClasses - 1 ? blocks_for(nb, kClassPerBlock[c]) : 0;
                     switch (c) {
-                        case 0: hipLaunchKernelGGL((k_front_fwd_lds<T, 8, 8, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
-                        case 1: hipLaunchKernelGGL((k_front_fwd_lds<T, 16, 16, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
-                        case 2: hipLaunchKernelGGL((k_front_fwd_lds<T, 32, 32, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
-                        case 3: hipLaunchKernelGGL((k_front_fwd_lds<T, 64, 64, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
-                        case 4: hipLaunchKernelGGL((k_front_fwd_lds<T, 128, 128, 128, TR>), dim3(gb), dim3(128), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        case 0: hipLaunchKernelGGL((k_front_fwd_lds<T, 4, 4, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        case 1: hipLaunchKernelGGL((k_front_fwd_lds<T, 8, 8, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        case 2: hipLaunchKernelGGL((k_front_fwd_lds<T, 16, 16, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        case 3: hipLaunchKernelGGL((k_front_fwd_lds<T, 32, 32, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        case 4: hipLaunchKernelGGL((k_front_fwd_lds<T, 64, 64, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
+                        case 5: hipLaunchKernelGGL((k_front_fwd_lds<T, 128, 128, 128, TR>), dim3(gb), dim3(128), 0, s, a, beg, nb, front, vec, rhs, sign); break;
                         default: hipLaunchKernelGGL((k_front_fwd<T, TR>), dim3(nb), dim3(256), 0, s, a, beg, front, vec, rhs, sign);
                     }
                 });
@@ -499,11 +741,12 @@ struct FrontRunner {
                 hook(kid + c, bytes, [&] {
                     int const gb = c < kNumClasses - 1 ? blocks_for(nb, kClassPerBlock[c]) : 0;
                     switch (c) {
-                        case 0: hipLaunchKernelGGL((k_front_bwd_lds<T, 8, 8, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
-                        case 1: hipLaunchKernelGGL((k_front_bwd_lds<T, 16, 16, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
-                        case 2: hipLaunchKernelGGL((k_front_bwd_lds<T, 32, 32, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
-                        case 3: hipLaunchKernelGGL((k_front_bwd_lds<T, 64, 64, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
-                        case 4: hipLaunchKernelGGL((k_front_bwd_lds<T, 128, 128, 128, TR>), dim3(gb), dim3(128), 0, s, a, beg, nb, front, vec, sol); break;
+                        case 0: hipLaunchKernelGGL((k_front_bwd_lds<T, 4, 4, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
+                        case 1: hipLaunchKernelGGL((k_front_bwd_lds<T, 8, 8, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
+                        case 2: hipLaunchKernelGGL((k_front_bwd_lds<T, 16, 16, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
+                        case 3: hipLaunchKernelGGL((k_front_bwd_lds<T, 32, 32, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
+                        case 4: hipLaunchKernelGGL((k_front_bwd_lds<T, 64, 64, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
+                        case 5: hipLaunchKernelGGL((k_front_bwd_lds<T, 128, 128, 128, TR>), dim3(gb), dim3(128), 0, s, a, beg, nb, front, vec, sol); break;
                         default: hipLaunchKernelGGL((k_front_bwd<T, TR>), dim3(nb), dim3(256), 0, s, a, beg, front, vec, sol);
                     }
                 });
@@ -594,6 +837,10 @@ struct gaphip_plan {
     gaphip::DBuf<double> front64, vec64;
     gaphip::DBuf<float> front32, vec32;
     gaphip::DevFronts fr;
+    gaphip::DBuf<int> bus_off_d;  // per-instance bus offsets for the grid schedule
+    int schedule = GAPHIP_SCHED_AUTO;
+
+    bool grid_schedule() const { return fr.fz_ok && schedule != GAPHIP_SCHED_LEVELS; }
 
     // host state
     bool sspec_dirty = true;
@@ -842,7 +1089,10 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
         int bus = P.node_bus[g][key >> 1];
         return P.var_off[g] + ((key & 1) ? P.np[g].mag[bus] : P.np[g].ang[bus]);
     };
-    pack_fronts(plans, ent_base, key_row, key_col, P.n_var, P.j_nnz, P.fr, s);
+    GridItems items{it_i, it_j, it_y, it_pos};
+    pack_fronts(plans, ent_base, key_row, key_col, P.n_var, P.j_nnz, P.fr, s, &items);
+    P.bus_off_d.upload(P.bus_off, s);
+    HIPCHK(hipStreamSynchronize(s));
 
     // per-bus / per-instance state and work arrays
     P.v.alloc(P.n_bus);
@@ -1007,6 +1257,98 @@ void run_solve(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front, T* v
     P.last_base = o.base_power;
 }
 
+// Grid schedule: one launch runs every Newton iteration of every network (one
+// workgroup per network, largest first).  Same prelude and the same
+// per-network results (iterations, converged flag, final mismatch, max |F|
+// rows) as the level-synchronous loop.
+constexpr int kGridWaves = 2;
+
+template <typename T, int NW>
+void launch_grid(gaphip_plan& P, int G, FusedArgs const& za, MisArgs const& ma, UpdArgs const& ua,
+                 FrontArgs const& fra, T* front, T* vec) {
+    size_t const shm = static_cast<size_t>(NW) * P.fr.fz_pool * sizeof(T);
+    if (shm > 64 * 1024) {
+        // above the default per-workgroup limit: gfx950 has 160 KiB of LDS per CU
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_nr_grid<T, NW>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(shm)) != hipSuccess)
+            throw Error(GAPHIP_E_CAPACITY, "grid schedule needs " + std::to_string(shm) +
+                                               " bytes of LDS per workgroup, more than the device allows");
+    }
+    hipLaunchKernelGGL((k_nr_grid<T, NW>), dim3(G), dim3(NW * kWave), shm, P.stream, za, ma, ua, fra, front, vec);
+}
+
+template <typename T>
+void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front, T* vec) {
+    hipStream_t s = P.stream;
+    int const G = P.n_inst;
+    int const max_it = o.max_iterations;
+    if (P.sspec_dirty || P.sspec_base != o.base_power) upload_sspec(P, o.base_power);
+    int rows = max_it + 1;
+    if (P.maxf_rows < rows) {
+        P.maxf.alloc(static_cast<size_t>(rows) * G);
+        P.active.alloc(rows);
+        P.maxf_rows = rows;
+    }
+    HIPCHK(hipMemsetAsync(P.maxf.p, 0, sizeof(unsigned long long) * rows * G, s));
+    HIPCHK(hipMemsetAsync(P.done.p, 0, sizeof(int) * G, s));
+    HIPCHK(hipMemsetAsync(P.iters.p, 0, sizeof(int) * G, s));
+    HIPCHK(hipMemsetAsync(P.lu_fail.p, 0, sizeof(int) * G, s));
+    HIPCHK(hipMemsetAsync(P.final_mis.p, 0, sizeof(double) * G, s));
+    if (P.prof.on) P.prof.reset();
+    long long const N = P.n_bus, Z = P.y_nnz;
+
+    YbusArgs ya{static_cast<int>(Z), P.y_seg.p, P.y_code.p, P.y_slot_inst.p, P.br_r.p, P.br_x.p, P.br_g.p,
+                P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.y_pu.p, nullptr};
+    launch(P, K_YBUS, 48.0 * Z, [&] { hipLaunchKernelGGL(k_ybus, dim3(blocks_for(Z, 256)), dim3(256), 0, s, ya); });
+    apply_given_y(P, o.base_power);
+    FlatArgs fa{static_cast<int>(N), P.flat.p, P.v.p};
+    launch(P, K_FLAT, 24.0 * N, [&] { hipLaunchKernelGGL(k_flat, dim3(blocks_for(N, 256)), dim3(256), 0, s, fa); });
+
+    if (max_it > 0) {
+        DevFronts const& d = P.fr;
+        FusedArgs za{G, max_it, o.tolerance, d.fz_order.p, P.bus_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
+                     d.fz_tasks.p, d.fz_pairs.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
+                     P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, P.dx.p};
+        MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
+                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p};
+        UpdArgs ua{static_cast<int>(N), P.bus_inst.p, P.done.p, P.lu_fail.p, P.ang.p, P.mag.p, P.dx.p, P.v.p};
+        FrontArgs fra = d.args(nullptr, P.lu_fail.p);
+        int nw = kGridWaves;
+        if (char const* e = std::getenv("GAPHIP_GRID_WAVES")) nw = std::atoi(e);
+        launch(P, K_FUSED, 0.0, [&] {
+            switch (nw) {
+                case 1: launch_grid<T, 1>(P, G, za, ma, ua, fra, front, vec); break;
+                case 2: launch_grid<T, 2>(P, G, za, ma, ua, fra, front, vec); break;
+                case 4: launch_grid<T, 4>(P, G, za, ma, ua, fra, front, vec); break;
+                default: launch_grid<T, 8>(P, G, za, ma, ua, fra, front, vec);
+            }
+        });
+    }
+    P.last_iters_run = max_it;
+    HIPCHK(hipStreamSynchronize(s));
+    if (P.prof.on) {
+        P.prof.collect();
+        // algorithmic bytes of the launch: per network, every full Newton
+        // iteration plus the final mismatch-only pass of a converged network
+        std::vector<int> its(G), done(G);
+        HIPCHK(hipMemcpy(its.data(), P.iters.p, sizeof(int) * G, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(done.data(), P.done.p, sizeof(int) * G, hipMemcpyDeviceToHost));
+        double const w = sizeof(T);
+        double total = 0.0;
+        for (int g = 0; g < G; ++g) {
+            double const n = P.bus_off[g + 1] - P.bus_off[g], z = P.y_off[g + 1] - P.y_off[g];
+            double const v = P.var_off[g + 1] - P.var_off[g];
+            double const mis = 20.0 * z + 56.0 * n + 8.0 * v;
+            double const full = mis + P.fr.fz_fixed[g] + w * P.fr.fz_welems[g] + (8.0 * v + 32.0 * n);
+            int const newton = done[g] ? its[g] - 1 : max_it;
+            total += newton * full + (done[g] ? mis : 0.0);
+        }
+        P.prof.bytes[K_FUSED] += total;
+    }
+    P.have_solution = true;
+    P.last_base = o.base_power;
+}
+
 // Adjoint at the converged state of the last solve: refactor J(x*) in FP64,
 // solve J^T lambda = dL/dx with the transposed front solves, then the bus and
 // branch gradient kernels.
@@ -1144,6 +1486,7 @@ int gaphip_analyze(int n_inst, const gaphip_net* nets, gaphip_plan_stats* o) {
         o->n_var = P.n_var;
         o->jac_nnz = P.j_nnz;
         for (auto const& L : P.lp) {
+            for (int m : L.f_m) o->front_hist[front_hist_bin(m)]++;
             o->n_fronts += L.n_fronts;
             o->n_levels = std::max<long long>(o->n_levels, L.n_levels);
             o->front_entries += L.front_entries;
@@ -1173,6 +1516,8 @@ int gaphip_plan_stats_get(const gaphip_plan* P, gaphip_plan_stats* o) {
         o->factor_entries = P->fr.factor_entries;
         o->max_front = P->fr.max_front;
         o->analysis_ms = P->analysis_ms;
+        for (auto const& L : P->lp)
+            for (int m : L.f_m) o->front_hist[front_hist_bin(m)]++;
     })
 }
 
@@ -1226,16 +1571,33 @@ int gaphip_plan_set_ybus(gaphip_plan* P, int inst, const int* rp, const int* ci,
     })
 }
 
+int gaphip_plan_set_schedule(gaphip_plan* P, int schedule, int* effective) {
+    GAPHIP_GUARD({
+        if (!P) throw Error(GAPHIP_E_ARG, "null plan");
+        if (schedule != GAPHIP_SCHED_AUTO && schedule != GAPHIP_SCHED_LEVELS && schedule != GAPHIP_SCHED_GRID)
+            throw Error(GAPHIP_E_ARG, "unknown schedule");
+        if (schedule == GAPHIP_SCHED_GRID && !P->fr.fz_ok)
+            throw Error(GAPHIP_E_ARG, "the grid schedule needs every front to have at most 64 slots (largest: " +
+                                          std::to_string(P->fr.max_front) + ")");
+        P->schedule = schedule;
+        if (effective) *effective = P->grid_schedule() ? GAPHIP_SCHED_GRID : GAPHIP_SCHED_LEVELS;
+    })
+}
+
 int gaphip_solve(gaphip_plan* P, const gaphip_solve_opts* o, double* v_ri, int* iterations, int* converged,
                  double* final_mismatch) {
     GAPHIP_GUARD({
         if (!P || !o) throw Error(GAPHIP_E_ARG, "null argument");
         if (o->max_iterations < 0) throw Error(GAPHIP_E_ARG, "max_iterations must be >= 0");
         if (!(o->base_power > 0.0)) throw Error(GAPHIP_E_ARG, "base_power must be > 0");
-        if (P->precision == GAPHIP_FP64)
-            run_solve<double>(*P, *o, P->jv64.p, P->front64.p, P->vec64.p);
-        else
-            run_solve<float>(*P, *o, P->jv32.p, P->front32.p, P->vec32.p);
+        bool const grid = P->grid_schedule();
+        if (P->precision == GAPHIP_FP64) {
+            if (grid) run_solve_fused<double>(*P, *o, P->jv64.p, P->front64.p, P->vec64.p);
+            else run_solve<double>(*P, *o, P->jv64.p, P->front64.p, P->vec64.p);
+        } else {
+            if (grid) run_solve_fused<float>(*P, *o, P->jv32.p, P->front32.p, P->vec32.p);
+            else run_solve<float>(*P, *o, P->jv32.p, P->front32.p, P->vec32.p);
+        }
         int const G = P->n_inst;
         std::vector<int> done(G), its(G);
         HIPCHK(hipMemcpy(done.data(), P->done.p, sizeof(int) * G, hipMemcpyDeviceToHost));

@@ -1,0 +1,437 @@
+// Grid schedule of the batched Newton-Raphson loop: one workgroup solves one
+// network of a batch from flat start to convergence inside a single launch.
+//
+// Per iteration the workgroup runs, with workgroup barriers between phases:
+//   mismatch + max|F| (cpu_newton_raphson.cpp:115-157, 243-337) -> convergence
+//   test -> multifrontal factorisation leaves -> root, with the Jacobian
+//   entries (:509-765) evaluated while each front is assembled and the forward
+//   substitution of -F folded into the elimination -> backward substitution
+//   root -> leaves -> damped polar update (:438-504).  The per-bus and
+//   per-pair arithmetic uses the rectangular forms below.
+//
+// A level of the network's elimination tree is a list of tasks; a task is up to
+// 64 / L fronts of one size class handled by one wavefront (L lanes per front,
+// lane t owning column t, the fronts staged in the wavefront's LDS region).
+// Assembly is driven by flat per-task lists built once per plan, so all 64
+// lanes gather the task's inputs together (two dependent loads per entry):
+//   pairs : one per Jacobian 2x2 block (buses i, j, Y slot, LDS position and
+//           which of the four entries exist), evaluated from Y, S and V;
+//   rhs   : -F of the pivot equations into the right-hand side;
+//   ext   : child Schur-complement entries, added into the front (LDS atomics,
+//           list order = child order, so every solve sums in the same order);
+//   vx    : child right-hand-side entries, added likewise;
+//   sol   : (backward) solution values of the structure slots.
+// Inside a task only wavefront-level synchronisation is needed; levels are
+// separated by workgroup barriers.
+#pragma once
+
+namespace gaphip {
+
+struct alignas(16) GridTask {
+    int front_beg, cls_cnt;  // first front id; class << 8 | front count
+    int pair_beg, pair_end;
+    int rhs_beg, rhs_end;
+    int ext_beg, ext_end;
+    int vx_beg, vx_end;
+    int sol_beg, sol_end;
+};
+
+struct FusedArgs {
+    int n_inst;
+    int max_it;
+    double tol;
+    const int* order;     // blockIdx -> network (largest first)
+    const int* bus_off;   // per network, n_inst + 1 entries
+    const int* lvl_beg;   // per network: its levels are [lvl_beg[g], lvl_beg[g+1]) of lvl_ptr
+    const int* lvl_ptr;   // per (network, level): tasks [lvl_ptr[e], lvl_ptr[e+1])
+    const GridTask* tasks;
+    const int4* pairs;    // (i, j, Y slot, LDS position | entry mask << 16)
+    const int2* rhs;      // (equation index, LDS position)
+    const int2* ext;      // (front storage index, LDS position)
+    const int2* vx;       // (vector index, LDS position)
+    const int2* sol;      // (variable index, LDS position)
+    int pool;             // LDS elements per wavefront
+    int ld4;              // leading dimension of the 64-slot class tiles (largest such front + 1)
+    unsigned long long* maxf;  // rows of per-network max |F| per iteration
+    int* done;
+    int* iters;
+    int* lu_fail;
+    double* final_mis;
+    double* dx;
+};
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// max over the groups of L lanes of one wavefront (every lane gets it)
+template <int L>
+__device__ inline int wave_group_max(int v) {
+    for (int off = L; off < kWave; off <<= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
+template <typename T>
+__device__ inline void lds_add(T* p, T v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// ---------------------------------------------------------------------------
+// Rectangular forms of the per-bus and per-pair arithmetic.  They equal the
+// reference's polar formulas (cpu_newton_raphson.cpp:438-504, 546-557,
+// 640-725) term by term and differ only in rounding; they avoid atan2, hypot
+// and sin/cos of arbitrary arguments, whose library code would dominate the
+// register budget of the grid kernel.
+
+// S_i = V_i conj((Y V)_i) and the P / Q mismatch rows; returns max |F| of the bus
+__device__ inline double grid_mismatch_bus(MisArgs const& a, int i) {
+#pragma clang fp contract(off)
+    double cr = 0.0, cim = 0.0;
+    for (int q = a.rp[i]; q < a.rp[i + 1]; ++q) {
+        double2 yv = a.y[q];
+        double2 vv = a.v[a.ci[q]];
+        cr += yv.x * vv.x - yv.y * vv.y;
+        cim += yv.x * vv.y + yv.y * vv.x;
+    }
+    double2 vi = a.v[i];
+    double sr = vi.x * cr + vi.y * cim;
+    double si = vi.y * cr - vi.x * cim;
+    a.s[i] = make_double2(sr, si);
+    double2 sp = a.sspec[i];
+    int pe = a.peq[i], qe = a.qeq[i];
+    double m = 0.0;
+    if (pe >= 0) {
+        double fp = sr - sp.x;
+        a.f[pe] = fp;
+        m = fabs(fp);
+    }
+    if (qe >= 0) {
+        double fq = si - sp.y;
+        a.f[qe] = fq;
+        m = fmax(m, fabs(fq));
+    }
+    return m;
+}
+
+// The Jacobian 2x2 block of Y pair (i, j) from rectangular V: with
+// e = V_i conj(V_j) = |V_i||V_j| (cos + j sin)(theta_i - theta_j),
+//   dP/dtheta_j = G Im e - B Re e,   dQ/dtheta_j = -(G Re e + B Im e),
+//   dP/d|V_j|  = (G Re e + B Im e) / |V_j|,   dQ/d|V_j| = (G Im e - B Re e) / |V_j|;
+// on the diagonal the reference's S-based forms.  A magnitude below 1e-6 is
+// replaced by 1 along the same direction, as the reference replaces |V| by 1.
+__device__ inline void jac_block_rect(bool diag, double2 yv, double2 vi, double2 vj, double2 si, double& pt,
+                                      double& pv, double& qt, double& qv) {
+    double const G = yv.x, B = yv.y;
+    double const ri = sqrt(vi.x * vi.x + vi.y * vi.y);
+    if (diag) {
+        double const Vi = ri < 1e-6 ? 1.0 : ri;
+        double const P = si.x, Q = si.y;
+        pt = -Q - B * Vi * Vi;
+        pv = (P / Vi) + G * Vi;
+        qt = P - G * Vi * Vi;
+        qv = (Q / Vi) - B * Vi;
+        return;
+    }
+    double rj = sqrt(vj.x * vj.x + vj.y * vj.y);
+    if (ri < 1e-6) vi = ri > 0.0 ? make_double2(vi.x / ri, vi.y / ri) : make_double2(1.0, 0.0);
+    if (rj < 1e-6) {
+        vj = rj > 0.0 ? make_double2(vj.x / rj, vj.y / rj) : make_double2(1.0, 0.0);
+        rj = 1.0;
+    }
+    double const re = vi.x * vj.x + vi.y * vj.y, im = vi.y * vj.x - vi.x * vj.y;
+    double const c = G * re + B * im, d = G * im - B * re;
+    pt = d;
+    qt = -c;
+    pv = c / rj;
+    qv = d / rj;
+}
+
+// sin and cos of |x| <= 0.15 by their Taylor series (truncation below 1e-20)
+__device__ inline void sincos_small(double x, double& sn, double& cs) {
+    double const x2 = x * x;
+    sn = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880 +
+                                                                                     x2 * (-1.0 / 39916800))))));
+    cs = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24 + x2 * (-1.0 / 720 + x2 * (1.0 / 40320 + x2 * (-1.0 / 3628800 +
+                                                                                           x2 * (1.0 / 479001600))))));
+}
+
+// Damped, clamped polar update: rotate V by the angle step, then rescale to the
+// clamped new magnitude.
+__device__ inline void grid_update_bus(UpdArgs const& a, int i, bool zero) {
+    double2 v = a.v[i];
+    int av = a.ang[i], mv = a.mag[i];
+    if (av >= 0) {
+        double ac = 0.9 * (zero ? 0.0 : a.dx[av]);
+        if (fabs(ac) > 0.15) ac = 0.15 * (ac > 0 ? 1.0 : -1.0);
+        double sn, cs;
+        sincos_small(ac, sn, cs);
+        v = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+    }
+    if (mv >= 0) {
+        double const old = sqrt(v.x * v.x + v.y * v.y);
+        double mc = 0.9 * (zero ? 0.0 : a.dx[mv]);
+        if (fabs(mc) > 0.08) mc = 0.08 * (mc > 0 ? 1.0 : -1.0);
+        double vm = old + mc;
+        vm = fmax(vm, 0.5);
+        vm = fmin(vm, 1.5);
+        v = old > 0.0 ? make_double2(v.x * (vm / old), v.y * (vm / old)) : make_double2(vm, 0.0);
+    }
+    a.v[i] = v;
+}
+
+// Chunks of 64 list entries each lane keeps in registers, so that all index
+// loads of a task are issued together, then all value loads; longer lists
+// finish in a loop.
+constexpr int kPairChunks = 1, kExtChunks = 2;
+
+// Assemble, factor and forward-eliminate up to kWave / L fronts of one class.
+// Stores the U rows (diagonal onwards), the Schur complement and the
+// eliminated right-hand side w of every front.
+template <typename T, int MM, int L>
+__device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, GridTask const& tk,
+                                 T* front, T* vec, T* pool, int ld) {
+    constexpr int G = kWave / L;
+    int const lane = threadIdx.x % kWave, gi = lane / L, t = lane % L;
+    int const rows = ld - 1, wbase = G * rows * ld, region = wbase + G * MM;
+    // indices
+    FrontMeta fm{};
+    if (gi < (tk.cls_cnt & 0xff)) fm = fa.meta[tk.front_beg + gi];
+    int4 pr[kPairChunks];
+#pragma unroll
+    for (int k = 0; k < kPairChunks; ++k) {
+        int const q = tk.pair_beg + k * kWave + lane;
+        pr[k] = q < tk.pair_end ? a.pairs[q] : make_int4(0, 0, 0, 0);
+    }
+    int2 ex[kExtChunks];
+#pragma unroll
+    for (int k = 0; k < kExtChunks; ++k) {
+        int const q = tk.ext_beg + k * kWave + lane;
+        ex[k] = q < tk.ext_end ? a.ext[q] : make_int2(0, -1);
+    }
+    int2 const rh = tk.rhs_beg + lane < tk.rhs_end ? a.rhs[tk.rhs_beg + lane] : make_int2(0, -1);
+    int2 const vv = tk.vx_beg + lane < tk.vx_end ? a.vx[tk.vx_beg + lane] : make_int2(0, -1);
+    // values
+    double2 yv[kPairChunks], vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
+#pragma unroll
+    for (int k = 0; k < kPairChunks; ++k) {
+        bool const live = (pr[k].w >> 16) != 0, diag = pr[k].x == pr[k].y;
+        double2 const z = make_double2(0.0, 0.0);
+        yv[k] = live ? ma.y[pr[k].z] : z;
+        vi[k] = live ? ma.v[pr[k].x] : z;
+        vj[k] = live && !diag ? ma.v[pr[k].y] : z;
+        si[k] = live && diag ? ma.s[pr[k].x] : z;
+    }
+    T exv[kExtChunks];
+#pragma unroll
+    for (int k = 0; k < kExtChunks; ++k) exv[k] = ex[k].y >= 0 ? front[ex[k].x] : T(0);
+    double const fv = rh.y >= 0 ? ma.f[rh.x] : 0.0;
+    T const vxv = vv.y >= 0 ? vec[vv.x] : T(0);
+    T* F = pool + gi * rows * ld;
+    T* w = pool + wbase + gi * MM;
+    wave_sync();  // the previous task of this wavefront has finished with the region
+    for (int e = lane; e < region; e += kWave) pool[e] = T(0);
+    wave_sync();
+    auto put_pair = [&](int4 const& p4, double2 y2, double2 pi2, double2 pj2, double2 s2) {
+        int const base = p4.w & 0xffff, mask = p4.w >> 16;
+        if (!mask) return;
+        double pt, pv, qt, qv;
+        jac_block_rect(p4.x == p4.y, y2, pi2, pj2, s2, pt, pv, qt, qv);
+        if (mask & 1) pool[base] = static_cast<T>(pt);
+        if (mask & 2) pool[base + 1] = static_cast<T>(pv);
+        if (mask & 4) pool[base + ld] = static_cast<T>(qt);
+        if (mask & 8) pool[base + ld + 1] = static_cast<T>(qv);
+    };
+#pragma unroll
+    for (int k = 0; k < kPairChunks; ++k) put_pair(pr[k], yv[k], vi[k], vj[k], si[k]);
+    for (int q = tk.pair_beg + kPairChunks * kWave + lane; q < tk.pair_end; q += kWave) {
+        int4 const p4 = a.pairs[q];
+        bool const diag = p4.x == p4.y;
+        double2 const z = make_double2(0.0, 0.0);
+        put_pair(p4, ma.y[p4.z], ma.v[p4.x], diag ? z : ma.v[p4.y], diag ? ma.s[p4.x] : z);
+    }
+    if (rh.y >= 0) pool[rh.y] = static_cast<T>(-1.0 * fv);
+    for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave) {
+        int2 const e = a.rhs[q];
+        pool[e.y] = static_cast<T>(-1.0 * ma.f[e.x]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < kExtChunks; ++k)
+        if (ex[k].y >= 0) lds_add(pool + ex[k].y, exv[k]);
+    for (int q = tk.ext_beg + kExtChunks * kWave + lane; q < tk.ext_end; q += kWave) {
+        int2 const e = a.ext[q];
+        lds_add(pool + e.y, front[e.x]);
+    }
+    if (vv.y >= 0) lds_add(pool + vv.y, vxv);
+    for (int q = tk.vx_beg + kWave + lane; q < tk.vx_end; q += kWave) {
+        int2 const e = a.vx[q];
+        lds_add(pool + e.y, vec[e.x]);
+    }
+    wave_sync();
+    int const m = fm.m, p = fm.p;
+    int const p_max = wave_group_max<L>(p);
+    for (int k = 0; k < p_max; ++k) {
+        T piv = k < p ? F[k * ld + k] : T(1);
+        bool const tiny = tabs(piv) < T(1e-14);
+        if (tiny) {
+            if (k < p && t == 0) fa.fail[fm.inst] = 1;
+            piv = T(1);
+        }
+        T const rp = T(1) / piv;
+        if (k < p && t > k && t < m) {
+            T const ukj = F[k * ld + t];
+            for (int i = k + 1; i < m; ++i) F[i * ld + t] -= (F[i * ld + k] * rp) * ukj;
+            w[t] -= (F[t * ld + k] * rp) * w[k];
+        }
+        wave_sync();
+        if (k < p && t == k)
+            for (int i = k + 1; i < m; ++i) F[i * ld + k] *= rp;
+    }
+    wave_sync();
+    if (t < m) {
+        T* Gm = front + fm.off;
+        for (int r = 0; r < m; ++r)
+            if (t >= (r < p ? r : p)) Gm[r * m + t] = F[r * ld + t];
+        vec[fm.voff + t] = w[t];
+    }
+}
+
+// Backward substitution of the pivot block of up to kWave / L fronts.
+template <typename T, int MM, int L>
+__device__ void grid_backward_task(FusedArgs const& a, FrontArgs const& fa, GridTask const& tk, T const* front,
+                                   T const* vec, double* sol, T* pool, int ld) {
+    constexpr int G = kWave / L, CH = MM < 16 ? MM : 16;
+    int const lane = threadIdx.x % kWave, gi = lane / L, t = lane % L;
+    int const rows = ld - 1, wbase = G * rows * ld;
+    FrontMeta fm{};
+    if (gi < (tk.cls_cnt & 0xff)) fm = fa.meta[tk.front_beg + gi];
+    int2 const so = tk.sol_beg + lane < tk.sol_end ? a.sol[tk.sol_beg + lane] : make_int2(0, -1);
+    T const sv = so.y >= 0 ? static_cast<T>(sol[so.x]) : T(0);
+    int const m = fm.m, p = fm.p;
+    T const wv = t < p ? vec[fm.voff + t] : T(0);
+    T* F = pool + gi * rows * ld;
+    T* w = pool + wbase + gi * MM;
+    const int* cols = fa.slot_col + fm.slot_beg;
+    wave_sync();  // the previous task of this wavefront has finished with the region
+    if (so.y >= 0) pool[so.y] = sv;
+    for (int q = tk.sol_beg + kWave + lane; q < tk.sol_end; q += kWave) {
+        int2 const e = a.sol[q];
+        pool[e.y] = static_cast<T>(sol[e.x]);
+    }
+    if (t < p) w[t] = wv;
+    load_rows<T, MM, L, CH>(F, ld, front + fm.off, m, p, t, t < m);
+    wave_sync();
+    if (t < p) {
+        T acc = w[t];
+        for (int q = p; q < m; ++q) acc -= F[t * ld + q] * w[q];
+        w[t] = acc;
+    }
+    wave_sync();
+    int const p_max = wave_group_max<L>(p);
+    for (int k = p_max - 1; k >= 0; --k) {
+        if (k < p) {
+            T const xk = w[k] / F[k * ld + k];
+            if (t < k) w[t] -= F[t * ld + k] * xk;
+            if (t == k) sol[cols[k]] = static_cast<double>(xk);
+        }
+        wave_sync();
+    }
+}
+
+template <typename T, bool BWD>
+__device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, GridTask const& tk,
+                                 T* front, T* vec, T* pool) {
+    switch (tk.cls_cnt >> 8) {
+        case 0:
+            if constexpr (BWD) grid_backward_task<T, 4, 4>(a, fa, tk, front, vec, a.dx, pool, 5);
+            else grid_factor_task<T, 4, 4>(a, fa, ma, tk, front, vec, pool, 5);
+            break;
+        case 1:
+            if constexpr (BWD) grid_backward_task<T, 8, 8>(a, fa, tk, front, vec, a.dx, pool, 9);
+            else grid_factor_task<T, 8, 8>(a, fa, ma, tk, front, vec, pool, 9);
+            break;
+        case 2:
+            if constexpr (BWD) grid_backward_task<T, 16, 16>(a, fa, tk, front, vec, a.dx, pool, 17);
+            else grid_factor_task<T, 16, 16>(a, fa, ma, tk, front, vec, pool, 17);
+            break;
+        case 3:
+            if constexpr (BWD) grid_backward_task<T, 32, 32>(a, fa, tk, front, vec, a.dx, pool, 33);
+            else grid_factor_task<T, 32, 32>(a, fa, ma, tk, front, vec, pool, 33);
+            break;
+        default:
+            if constexpr (BWD) grid_backward_task<T, 64, 64>(a, fa, tk, front, vec, a.dx, pool, a.ld4);
+            else grid_factor_task<T, 64, 64>(a, fa, ma, tk, front, vec, pool, a.ld4);
+    }
+}
+
+// This wavefront's tasks q, q + NW, ... < qe of one level; the next
+// descriptor is loaded while the current task runs.
+template <typename T, bool BWD, int NW>
+__device__ inline void run_level(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int q, int qe, T* front,
+                                 T* vec, T* pool) {
+    if (q >= qe) return;
+    GridTask nxt = a.tasks[q];
+    for (; q < qe; q += NW) {
+        GridTask const cur = nxt;
+        if (q + NW < qe) nxt = a.tasks[q + NW];
+        grid_task<T, BWD>(a, fa, ma, cur, front, vec, pool);
+    }
+}
+
+template <int NW>
+__device__ inline double block_max(double m, double* red) {
+    for (int off = kWave / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    int const wave = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) red[wave] = m;
+    __syncthreads();
+    m = red[0];
+    for (int k = 1; k < NW; ++k) m = fmax(m, red[k]);
+    __syncthreads();
+    return m;
+}
+
+// One workgroup per network; dynamic LDS = NW * pool elements of T.
+template <typename T, int NW>
+__global__ void __launch_bounds__(NW* kWave) k_nr_grid(FusedArgs a, MisArgs ma, UpdArgs ua, FrontArgs fa, T* front,
+                                                        T* vec) {
+    constexpr int NT = NW * kWave;
+    extern __shared__ __align__(16) unsigned char grid_smem[];
+    __shared__ double red[NW];
+    int const g = a.order[blockIdx.x];
+    int const b0 = a.bus_off[g], b1 = a.bus_off[g + 1];
+    int const e0 = a.lvl_beg[g], e1 = a.lvl_beg[g + 1];
+    int const tid = threadIdx.x, wave = tid / kWave;
+    T* pool = reinterpret_cast<T*>(grid_smem) + wave * a.pool;
+    for (int it = 0; it < a.max_it; ++it) {
+        double m = 0.0;
+        for (int i = b0 + tid; i < b1; i += NT) m = fmax(m, grid_mismatch_bus(ma, i));
+        m = block_max<NW>(m, red);  // its barriers also publish S and F
+        if (tid == 0) {
+            a.maxf[static_cast<long long>(it) * a.n_inst + g] = dbits(m);
+            a.final_mis[g] = m;
+            a.lu_fail[g] = 0;
+        }
+        if (m < a.tol) {
+            if (tid == 0) {
+                a.done[g] = 1;
+                a.iters[g] = it + 1;
+            }
+            break;
+        }
+        for (int e = e0; e < e1; ++e) {
+            run_level<T, false, NW>(a, fa, ma, a.lvl_ptr[e] + wave, a.lvl_ptr[e + 1], front, vec, pool);
+            __syncthreads();
+        }
+        for (int e = e1 - 1; e >= e0; --e) {
+            run_level<T, true, NW>(a, fa, ma, a.lvl_ptr[e] + wave, a.lvl_ptr[e + 1], front, vec, pool);
+            __syncthreads();
+        }
+        bool const zero = a.lu_fail[g] != 0;
+        for (int i = b0 + tid; i < b1; i += NT) grid_update_bus(ua, i, zero);
+        __syncthreads();
+    }
+}
+
+}  // namespace gaphip

@@ -112,42 +112,46 @@ struct MisArgs {
 
 __device__ inline unsigned long long dbits(double x) { return __double_as_longlong(x); }
 
-__global__ void __launch_bounds__(256) k_mismatch(MisArgs a) {
+// per bus: S, (|V|, arg V) and the P / Q mismatch rows; returns this bus's max |F|
+__device__ inline double mismatch_bus(MisArgs const& a, int i) {
 #pragma clang fp contract(off)
+    double cr = 0.0, cim = 0.0;
+    for (int q = a.rp[i]; q < a.rp[i + 1]; ++q) {
+        double2 yv = a.y[q];
+        double2 vv = a.v[a.ci[q]];
+        double pr = yv.x * vv.x - yv.y * vv.y;
+        double pi = yv.x * vv.y + yv.y * vv.x;
+        cr += pr;
+        cim += pi;
+    }
+    double2 vi = a.v[i];
+    double c = cr, d = -cim;
+    double sr = vi.x * c - vi.y * d;
+    double si = vi.x * d + vi.y * c;
+    a.s[i] = make_double2(sr, si);
+    a.vpol[i] = make_double2(hypot(vi.x, vi.y), atan2(vi.y, vi.x));
+    double2 sp = a.sspec[i];
+    int pe = a.peq[i], qe = a.qeq[i];
+    double m = 0.0;
+    if (pe >= 0) {
+        double fp = sr - sp.x;
+        a.f[pe] = fp;
+        m = fabs(fp);
+    }
+    if (qe >= 0) {
+        double fq = si - sp.y;
+        a.f[qe] = fq;
+        m = fmax(m, fabs(fq));
+    }
+    return m;
+}
+
+__global__ void __launch_bounds__(256) k_mismatch(MisArgs a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool live = i < a.n_bus;
     int inst = live ? a.bus_inst[i] : -1;
     if (live && a.done[inst]) live = false;
-    double m = 0.0;
-    if (live) {
-        double cr = 0.0, cim = 0.0;
-        for (int q = a.rp[i]; q < a.rp[i + 1]; ++q) {
-            double2 yv = a.y[q];
-            double2 vv = a.v[a.ci[q]];
-            double pr = yv.x * vv.x - yv.y * vv.y;
-            double pi = yv.x * vv.y + yv.y * vv.x;
-            cr += pr;
-            cim += pi;
-        }
-        double2 vi = a.v[i];
-        double c = cr, d = -cim;
-        double sr = vi.x * c - vi.y * d;
-        double si = vi.x * d + vi.y * c;
-        a.s[i] = make_double2(sr, si);
-        a.vpol[i] = make_double2(hypot(vi.x, vi.y), atan2(vi.y, vi.x));
-        double2 sp = a.sspec[i];
-        int pe = a.peq[i], qe = a.qeq[i];
-        if (pe >= 0) {
-            double fp = sr - sp.x;
-            a.f[pe] = fp;
-            m = fabs(fp);
-        }
-        if (qe >= 0) {
-            double fq = si - sp.y;
-            a.f[qe] = fq;
-            m = fmax(m, fabs(fq));
-        }
-    }
+    double m = live ? mismatch_bus(a, i) : 0.0;
     // wave-uniform instance: reduce in registers, one atomic per wave
     int lane = threadIdx.x & (kWave - 1);
     int inst0 = __shfl(inst, 0);
@@ -204,26 +208,19 @@ struct JacArgs {
     const double2* vpol;
 };
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_jacobian(JacArgs a, T* jv) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n_items) return;
-    int i = a.it_i[t], j = a.it_j[t];
-    if (a.done[a.bus_inst[i]]) return;
-    double2 yv = a.y[a.it_y[t]];
+// dP/dtheta_j, dP/d|V_j|, dQ/dtheta_j, dQ/d|V_j| of bus i from its Y value yv,
+// (|V|, arg V) of buses i and j, and S_i (used on the diagonal)
+__device__ inline void jac_block_v(bool diag, double2 yv, double2 pi, double2 pj, double2 si, double& pt,
+                                   double& pv, double& qt, double& qv) {
     double G = yv.x, B = yv.y;
-    double2 pi = a.vpol[i];
     double Vi = pi.x < 1e-6 ? 1.0 : pi.x;
-    double pt, pv, qt, qv;
-    if (i == j) {
-        double2 si = a.s[i];
+    if (diag) {
         double P = si.x, Q = si.y;
         pt = -Q - B * Vi * Vi;
         pv = (P / Vi) + G * Vi;
         qt = P - G * Vi * Vi;
         qv = (Q / Vi) - B * Vi;
     } else {
-        double2 pj = a.vpol[j];
         double Vj = pj.x < 1e-6 ? 1.0 : pj.x;
         double th = pi.y - pj.y;
         double sn, cs;
@@ -233,11 +230,32 @@ __global__ void __launch_bounds__(256) k_jacobian(JacArgs a, T* jv) {
         qt = -Vi * Vj * (G * cs + B * sn);
         qv = Vi * (G * sn - B * cs);
     }
+}
+
+__device__ inline void jac_block(const double2* y, const double2* s, const double2* vpol, int i, int j, int yi,
+                                 double& pt, double& pv, double& qt, double& qv) {
+    bool const diag = i == j;
+    jac_block_v(diag, y[yi], vpol[i], diag ? make_double2(0.0, 0.0) : vpol[j], diag ? s[i] : make_double2(0.0, 0.0),
+                pt, pv, qt, qv);
+}
+
+template <typename T>
+__device__ inline void jacobian_item(JacArgs const& a, int t, T* jv) {
+    double pt, pv, qt, qv;
+    jac_block(a.y, a.s, a.vpol, a.it_i[t], a.it_j[t], a.it_y[t], pt, pv, qt, qv);
     const int* p4 = a.it_pos + 4 * t;
     if (p4[0] >= 0) jv[p4[0]] = static_cast<T>(pt);
     if (p4[1] >= 0) jv[p4[1]] = static_cast<T>(pv);
     if (p4[2] >= 0) jv[p4[2]] = static_cast<T>(qt);
     if (p4[3] >= 0) jv[p4[3]] = static_cast<T>(qv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_jacobian(JacArgs a, T* jv) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n_items) return;
+    if (a.done[a.bus_inst[a.it_i[t]]]) return;
+    jacobian_item<T>(a, t, jv);
 }
 
 // ------------------------------------------------------- multifrontal LU
@@ -823,13 +841,8 @@ struct UpdArgs {
     const double* dx;
     double2* v;
 };
-__global__ void __launch_bounds__(256) k_update(UpdArgs a) {
+__device__ inline void update_bus(UpdArgs const& a, int i, bool zero) {
 #pragma clang fp contract(off)
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n_bus) return;
-    int inst = a.bus_inst[i];
-    if (a.done[inst]) return;
-    bool zero = a.fail[inst] != 0;
     double2 v = a.v[i];
     int av = a.ang[i], mv = a.mag[i];
     if (av >= 0) {
@@ -850,6 +863,14 @@ __global__ void __launch_bounds__(256) k_update(UpdArgs a) {
         v = make_double2(vm * cos(va), vm * sin(va));
     }
     a.v[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_update(UpdArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_bus) return;
+    int inst = a.bus_inst[i];
+    if (a.done[inst]) return;
+    update_bus(a, i, a.fail[inst] != 0);
 }
 
 }  // namespace gaphip

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define GAPHIP_ABI_VERSION 1
+#define GAPHIP_ABI_VERSION 2
 
 enum {
     GAPHIP_OK = 0,
@@ -100,6 +100,8 @@ typedef struct gaphip_plan_stats {
     long long n_bus, n_branch, ybus_nnz, n_var, jac_nnz;
     long long n_fronts, n_levels, front_entries, factor_entries, max_front;
     double analysis_ms;
+    /* fronts by size: at most 2, 4, 8, 16, 32, 64, 128 slots, and above */
+    long long front_hist[8];
 } gaphip_plan_stats;
 
 int gaphip_plan_create(int n_inst, const gaphip_net* nets, int precision, gaphip_plan** out);
@@ -123,6 +125,16 @@ int gaphip_plan_set_branch_rx(gaphip_plan* plan, int inst, const double* r, cons
  * Passing row_ptr = NULL returns the instance to on-device assembly. */
 int gaphip_plan_set_ybus(gaphip_plan* plan, int inst, const int* row_ptr, const int* col_idx,
                          const double* values_ri, int nnz, int per_unit);
+
+/* Device schedule of the Newton loop.  LEVELS: one launch per phase and per
+ * (elimination-tree level, front size class) over all networks, host check of
+ * convergence every iteration.  GRID: one workgroup per network runs every
+ * iteration of that network inside a single launch (needs every front of the
+ * plan to have at most 64 slots).  AUTO (the default) picks GRID when the plan
+ * allows it.  `effective` (may be NULL) receives the schedule solves will use.
+ * No counterpart in the reference: its CPU solver has no device schedule. */
+enum { GAPHIP_SCHED_AUTO = 0, GAPHIP_SCHED_LEVELS = 1, GAPHIP_SCHED_GRID = 2 };
+int gaphip_plan_set_schedule(gaphip_plan* plan, int schedule, int* effective);
 
 typedef struct gaphip_solve_opts {
     double tolerance;   /* PowerFlowConfig::tolerance (powerflow_interface.h:14)       */

@@ -62,10 +62,13 @@ def test_mismatch_at_reference_solution(device, oracle, name):
         assert np.abs(f).max() < float(g["tolerance"])
 
 
+@pytest.mark.parametrize("schedule", ["auto", "levels"])
 @pytest.mark.parametrize("name", CASES)
-def test_solve_fp64_matches_reference(device, name):
+def test_solve_fp64_matches_reference(device, name, schedule):
     net, g = load_golden(name)
-    plan = capi.Plan([net])
+    plan = capi.Plan([net], schedule=schedule)
+    if schedule == "auto":
+        assert plan.schedule == ("grid" if plan.stats()["max_front"] <= 64 else "levels")
     r = plan.solve(float(g["tolerance"]), int(g["max_iterations"]), float(g["base_power"]))
     print(name, "max|dV| vs reference:", np.abs(r["v"][0] - g["v"]).max())
     assert int(r["iterations"][0]) == int(g["iterations"])
@@ -84,13 +87,14 @@ def test_solve_pgm_fixture_voltages(device, name):
     assert np.abs(np.abs(r["v"][0]) - g["pgm_u_pu"]).max() < 5e-6
 
 
-def test_batch_equals_single(device):
+@pytest.mark.parametrize("schedule", ["auto", "levels"])
+def test_batch_equals_single(device, schedule):
     names = [n for n in CASES if not n.startswith("nonconvergent")]
     nets = [load_golden(n)[0] for n in names]
     # one tolerance / base for the whole batch: use the mesh settings
-    batch = capi.Plan(nets).solve(1e-8, 100, 100e6)
+    batch = capi.Plan(nets, schedule=schedule).solve(1e-8, 100, 100e6)
     for k, net in enumerate(nets):
-        single = capi.Plan([net]).solve(1e-8, 100, 100e6)
+        single = capi.Plan([net], schedule=schedule).solve(1e-8, 100, 100e6)
         assert batch["iterations"][k] == single["iterations"][0]
         assert np.array_equal(batch["v"][k], single["v"][0]), names[k]
 
@@ -116,14 +120,76 @@ def test_large_grid_fixed_point(device, oracle):
     assert np.abs(f).max() < 1e-8
 
 
-def test_mixed_precision_close_to_fp64(device):
+@pytest.mark.parametrize("schedule", ["auto", "levels"])
+def test_mixed_precision_close_to_fp64(device, schedule):
     nets = [gridgen.small_mesh(n, seed=s) for n, s in [(200, 0), (800, 4), (3000, 9)]]
-    a = capi.Plan(nets, precision="fp64").solve(1e-8, 100, 100e6)
-    b = capi.Plan(nets, precision="mixed").solve(1e-8, 100, 100e6)
+    a = capi.Plan(nets, precision="fp64", schedule=schedule).solve(1e-8, 100, 100e6)
+    b = capi.Plan(nets, precision="mixed", schedule=schedule).solve(1e-8, 100, 100e6)
     assert b["converged"].all()
     for k in range(len(nets)):
         assert np.abs(a["v"][k] - b["v"][k]).max() <= V_TOL
         assert abs(int(a["iterations"][k]) - int(b["iterations"][k])) <= 1
+
+
+def _batch(count, seed):
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(50), np.log(3000), count)).astype(int)
+    return [gridgen.small_mesh(int(n), seed=seed + k, n_extra=40,
+                               load_factor=float(rng.uniform(0.35, 0.65))) for k, n in enumerate(sizes)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_grid_schedule_matches_levels(device, precision):
+    """One workgroup per network in one launch gives the same Newton iterates
+    as the level-synchronous schedule: same iteration counts and trajectories,
+    voltages equal to rounding (the grid schedule evaluates the Jacobian and
+    the update in rectangular form, equal to the polar formulas up to rounding)."""
+    nets = _batch(24, 500)
+    g = capi.Plan(nets, precision=precision, schedule="grid")
+    lv = capi.Plan(nets, precision=precision, schedule="levels")
+    assert g.schedule == "grid" and lv.schedule == "levels"
+    rg = g.solve(1e-8, 100, 100e6)
+    rl = lv.solve(1e-8, 100, 100e6)
+    assert rg["converged"].all() and rl["converged"].all()
+    assert np.array_equal(rg["iterations"], rl["iterations"])
+    for k in range(len(nets)):
+        assert np.abs(rg["v"][k] - rl["v"][k]).max() <= (1e-12 if precision == "fp64" else 1e-10)
+        tg, tl = g.trajectory(k), lv.trajectory(k)
+        assert len(tg) == len(tl) == rg["iterations"][k]
+        # the FP32 factors carry the schedules' last-bit Jacobian differences
+        # into later iterates at FP32 level
+        assert np.allclose(tg, tl, rtol=1e-6 if precision == "fp64" else 1e-3, atol=1e-13)
+    # repeated solves on the same plan give the same answer
+    rg2 = g.solve(1e-8, 100, 100e6)
+    assert all(np.array_equal(a, b) for a, b in zip(rg["v"], rg2["v"]))
+
+
+def test_grid_schedule_vs_oracle(device, oracle):
+    nets = _batch(6, 900)
+    r = capi.Plan(nets, schedule="grid").solve(1e-8, 100, 100e6)
+    for k, net in enumerate(nets):
+        o = oracle.solve(net, tol=1e-8, max_it=100, j_call=-1)
+        assert o["converged"] and r["converged"][k]
+        assert int(r["iterations"][k]) == o["iterations"]
+        assert np.abs(r["v"][k] - o["v"]).max() <= V_TOL
+
+
+def test_grid_schedule_limits(device):
+    # iteration cap reached: same bookkeeping as the level schedule
+    nets = _batch(4, 77)
+    for sched in ("grid", "levels"):
+        r = capi.Plan(nets, schedule=sched).solve(1e-8, 2, 100e6)
+        assert not r["converged"].any() and (r["iterations"] == 2).all() and (r["final_mismatch"] > 1e-8).all()
+        r0 = capi.Plan(nets, schedule=sched).solve(1e-8, 0, 100e6)
+        assert not r0["converged"].any() and (r0["iterations"] == 0).all()
+    # fronts above 64 slots cannot use the grid schedule
+    big = gridgen.large_mesh(3000, seed=2, n_extra=900, n_trafo=300, n_shunt=60)
+    p = capi.Plan([big])
+    if p.stats()["max_front"] > 64:
+        assert p.schedule == "levels"
+        with pytest.raises(capi.GapHipError) as e:
+            p.set_schedule("grid")
+        assert e.value.code == capi.E_ARG
 
 
 @pytest.mark.parametrize("n", [10, 60])
