@@ -59,6 +59,13 @@ class PlanStats(C.Structure):
         return d
 
 
+class Tuning(C.Structure):
+    _fields_ = [("grid_waves", C.c_int), ("grid_big_waves", C.c_int), ("grid_big_work", C.c_longlong)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int), ("base_power", C.c_double)]
 
@@ -77,6 +84,8 @@ SIGNATURES = {
     "gaphip_plan_set_branch_rx": (C.c_int, [C.c_void_p, C.c_int, _d, _d]),
     "gaphip_plan_set_ybus": (C.c_int, [C.c_void_p, C.c_int, _i, _i, _d, C.c_int, C.c_int]),
     "gaphip_plan_set_schedule": (C.c_int, [C.c_void_p, C.c_int, _i]),
+    "gaphip_plan_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "gaphip_plan_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "gaphip_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), _d, _i, _i, _d]),
     "gaphip_trajectory": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_int, _i]),
     "gaphip_jacobian": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_double, _i, _i, _i, _d, C.c_int, _i]),
@@ -217,6 +226,19 @@ class Plan:
     @property
     def handle(self):
         return self._h
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().gaphip_plan_get_tuning(self._h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, grid_waves=None, grid_big_waves=None, grid_big_work=None) -> dict:
+        cur = self.tuning()
+        t = Tuning(cur["grid_waves"] if grid_waves is None else int(grid_waves),
+                   cur["grid_big_waves"] if grid_big_waves is None else int(grid_big_waves),
+                   cur["grid_big_work"] if grid_big_work is None else int(grid_big_work))
+        _check(lib().gaphip_plan_set_tuning(self._h, C.byref(t)), "set_tuning")
+        return t.as_dict()
 
     def stats(self) -> dict:
         st = PlanStats()

@@ -227,9 +227,12 @@ struct DevFronts {
     DBuf<int> fz_order, fz_lvl_beg, fz_lvl_ptr;
     DBuf<GridTask> fz_tasks;
     DBuf<int4> fz_pairs;
+    DBuf<double2> fz_ypair;  // Y value of each pair, refreshed at every solve
+    long long fz_npairs = 0;
     DBuf<int2> fz_rhs, fz_ext, fz_vx, fz_sol;
     long long fz_list_bytes = 0;
     std::vector<double> fz_fixed, fz_welems;  // per instance, per Newton iteration (see build_grid)
+    std::vector<long long> fz_work;  // front slots + factor entries of fz_order's networks (descending)
 
     BigMaps big() const { return BigMaps{bigbase.p, arow.p, crow.p, crow_ci.p}; }
 
@@ -430,6 +433,8 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
         return d.inst_factor[x] + d.inst_slots[x] > d.inst_factor[y] + d.inst_slots[y];
     });
+    d.fz_work.resize(G);
+    for (int k = 0; k < G; ++k) d.fz_work[k] = d.inst_factor[order[k]] + d.inst_slots[order[k]];
     auto nonempty = [](auto& v, auto z) {
         if (v.empty()) v.push_back(z);
     };
@@ -446,6 +451,8 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     d.fz_lvl_ptr.upload(lvl_ptr, s);
     d.fz_tasks.upload(tasks, s);
     d.fz_pairs.upload(pairs, s);
+    d.fz_npairs = static_cast<long long>(pairs.size());
+    d.fz_ypair.alloc(d.fz_npairs);
     d.fz_rhs.upload(rhs, s);
     d.fz_ext.upload(ext, s);
     d.fz_vx.upload(vx, s);
@@ -839,6 +846,7 @@ struct gaphip_plan {
     gaphip::DevFronts fr;
     gaphip::DBuf<int> bus_off_d;  // per-instance bus offsets for the grid schedule
     int schedule = GAPHIP_SCHED_AUTO;
+    gaphip_tuning tuning{2, 8, -1};
 
     bool grid_schedule() const { return fr.fz_ok && schedule != GAPHIP_SCHED_LEVELS; }
 
@@ -854,10 +862,15 @@ struct gaphip_plan {
     gaphip::Profiler prof;
     int* pinned_active = nullptr;
     hipEvent_t ev = nullptr;
+    hipStream_t stream2 = nullptr;  // second stream of the grid schedule (default priority)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     ~gaphip_plan() {
         if (pinned_active) (void)hipHostFree(pinned_active);
         if (ev) (void)hipEventDestroy(ev);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -947,6 +960,9 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&P.pinned_active), sizeof(int)));
     HIPCHK(hipEventCreateWithFlags(&P.ev, hipEventDisableTiming));
+    HIPCHK(hipStreamCreateWithFlags(&P.stream2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
     hipStream_t s = P.stream;
 
     std::vector<int> bus_inst(P.n_bus), ang(P.n_bus), mag(P.n_bus), peq(P.n_bus), qeq(P.n_bus);
@@ -1261,11 +1277,29 @@ void run_solve(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front, T* v
 // workgroup per network, largest first).  Same prelude and the same
 // per-network results (iterations, converged flag, final mismatch, max |F|
 // rows) as the level-synchronous loop.
-constexpr int kGridWaves = 2;
+// Networks whose elimination trees hold at least tuning.grid_big_work front
+// slots and factor entries run with tuning.grid_big_waves wavefronts per
+// workgroup (their levels are wide), the rest with tuning.grid_waves; the two
+// launches run concurrently.
+template <typename T, int NW>
+void launch_grid(gaphip_plan& P, hipStream_t st, int first, int count, FusedArgs za, MisArgs const& ma,
+                 UpdArgs const& ua, FrontArgs const& fra, T* front, T* vec);
+
+template <typename T>
+void launch_grid_n(int nw, gaphip_plan& P, hipStream_t st, int first, int count, FusedArgs const& za,
+                   MisArgs const& ma, UpdArgs const& ua, FrontArgs const& fra, T* front, T* vec) {
+    switch (nw) {
+        case 1: launch_grid<T, 1>(P, st, first, count, za, ma, ua, fra, front, vec); break;
+        case 2: launch_grid<T, 2>(P, st, first, count, za, ma, ua, fra, front, vec); break;
+        case 4: launch_grid<T, 4>(P, st, first, count, za, ma, ua, fra, front, vec); break;
+        default: launch_grid<T, 8>(P, st, first, count, za, ma, ua, fra, front, vec);
+    }
+}
 
 template <typename T, int NW>
-void launch_grid(gaphip_plan& P, int G, FusedArgs const& za, MisArgs const& ma, UpdArgs const& ua,
-                 FrontArgs const& fra, T* front, T* vec) {
+void launch_grid(gaphip_plan& P, hipStream_t st, int first, int count, FusedArgs za, MisArgs const& ma,
+                 UpdArgs const& ua, FrontArgs const& fra, T* front, T* vec) {
+    if (count <= 0) return;
     size_t const shm = static_cast<size_t>(NW) * P.fr.fz_pool * sizeof(T);
     if (shm > 64 * 1024) {
         // above the default per-workgroup limit: gfx950 has 160 KiB of LDS per CU
@@ -1274,7 +1308,8 @@ void launch_grid(gaphip_plan& P, int G, FusedArgs const& za, MisArgs const& ma, 
             throw Error(GAPHIP_E_CAPACITY, "grid schedule needs " + std::to_string(shm) +
                                                " bytes of LDS per workgroup, more than the device allows");
     }
-    hipLaunchKernelGGL((k_nr_grid<T, NW>), dim3(G), dim3(NW * kWave), shm, P.stream, za, ma, ua, fra, front, vec);
+    za.order_off = first;
+    hipLaunchKernelGGL((k_nr_grid<T, NW>), dim3(count), dim3(NW * kWave), shm, st, za, ma, ua, fra, front, vec);
 }
 
 template <typename T>
@@ -1306,21 +1341,31 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
 
     if (max_it > 0) {
         DevFronts const& d = P.fr;
-        FusedArgs za{G, max_it, o.tolerance, d.fz_order.p, P.bus_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
-                     d.fz_tasks.p, d.fz_pairs.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
+        hipLaunchKernelGGL(k_gather_ypair, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
+                           static_cast<int>(d.fz_npairs), d.fz_pairs.p, P.y_pu.p, d.fz_ypair.p);
+        FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
+                     d.fz_tasks.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
                      P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, P.dx.p};
         MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
                    P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p};
         UpdArgs ua{static_cast<int>(N), P.bus_inst.p, P.done.p, P.lu_fail.p, P.ang.p, P.mag.p, P.dx.p, P.v.p};
         FrontArgs fra = d.args(nullptr, P.lu_fail.p);
-        int nw = kGridWaves;
-        if (char const* e = std::getenv("GAPHIP_GRID_WAVES")) nw = std::atoi(e);
+        gaphip_tuning const& tu = P.tuning;
+        int nbig = 0;
+        if (tu.grid_big_work >= 0)
+            while (nbig < G && d.fz_work[nbig] >= tu.grid_big_work) ++nbig;
         launch(P, K_FUSED, 0.0, [&] {
-            switch (nw) {
-                case 1: launch_grid<T, 1>(P, G, za, ma, ua, fra, front, vec); break;
-                case 2: launch_grid<T, 2>(P, G, za, ma, ua, fra, front, vec); break;
-                case 4: launch_grid<T, 4>(P, G, za, ma, ua, fra, front, vec); break;
-                default: launch_grid<T, 8>(P, G, za, ma, ua, fra, front, vec);
+            if (nbig > 0 && nbig < G) {
+                // big networks on the side stream, concurrently with the rest
+                HIPCHK(hipEventRecord(P.ev_fork, s));
+                HIPCHK(hipStreamWaitEvent(P.stream2, P.ev_fork, 0));
+                launch_grid_n<T>(tu.grid_big_waves, P, P.stream2, 0, nbig, za, ma, ua, fra, front, vec);
+                HIPCHK(hipEventRecord(P.ev_join, P.stream2));
+                launch_grid_n<T>(tu.grid_waves, P, s, nbig, G - nbig, za, ma, ua, fra, front, vec);
+                HIPCHK(hipStreamWaitEvent(s, P.ev_join, 0));
+            } else {
+                launch_grid_n<T>(nbig == G ? tu.grid_big_waves : tu.grid_waves, P, s, 0, G, za, ma, ua, fra, front,
+                                 vec);
             }
         });
     }
@@ -1581,6 +1626,23 @@ int gaphip_plan_set_schedule(gaphip_plan* P, int schedule, int* effective) {
                                           std::to_string(P->fr.max_front) + ")");
         P->schedule = schedule;
         if (effective) *effective = P->grid_schedule() ? GAPHIP_SCHED_GRID : GAPHIP_SCHED_LEVELS;
+    })
+}
+
+int gaphip_plan_get_tuning(const gaphip_plan* P, gaphip_tuning* out) {
+    GAPHIP_GUARD({
+        if (!P || !out) throw Error(GAPHIP_E_ARG, "null argument");
+        *out = P->tuning;
+    })
+}
+
+int gaphip_plan_set_tuning(gaphip_plan* P, const gaphip_tuning* t) {
+    GAPHIP_GUARD({
+        if (!P || !t) throw Error(GAPHIP_E_ARG, "null argument");
+        auto ok = [](int w) { return w == 1 || w == 2 || w == 4 || w == 8; };
+        if (!ok(t->grid_waves) || !ok(t->grid_big_waves))
+            throw Error(GAPHIP_E_ARG, "wavefronts per workgroup must be 1, 2, 4 or 8");
+        P->tuning = *t;
     })
 }
 

@@ -40,12 +40,14 @@ struct FusedArgs {
     int n_inst;
     int max_it;
     double tol;
-    const int* order;     // blockIdx -> network (largest first)
+    int order_off;        // this launch covers order[order_off + blockIdx.x]
+    const int* order;     // networks, largest first
     const int* bus_off;   // per network, n_inst + 1 entries
     const int* lvl_beg;   // per network: its levels are [lvl_beg[g], lvl_beg[g+1]) of lvl_ptr
     const int* lvl_ptr;   // per (network, level): tasks [lvl_ptr[e], lvl_ptr[e+1])
     const GridTask* tasks;
     const int4* pairs;    // (i, j, Y slot, LDS position | entry mask << 16)
+    const double2* ypair; // per-unit Y value of each pair, in pair order (gathered once per solve)
     const int2* rhs;      // (equation index, LDS position)
     const int2* ext;      // (front storage index, LDS position)
     const int2* vx;       // (vector index, LDS position)
@@ -73,6 +75,26 @@ __device__ inline int wave_group_max(int v) {
     return v;
 }
 
+// Plan lists, and front data read once per iteration, stream past the caches
+// so that the per-network bus arrays stay resident.
+template <typename T>
+__device__ inline T ld_stream(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+__device__ inline int4 ld_stream(const int4* p) {
+    const int* q = reinterpret_cast<const int*>(p);
+    return make_int4(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1), __builtin_nontemporal_load(q + 2),
+                     __builtin_nontemporal_load(q + 3));
+}
+__device__ inline int2 ld_stream(const int2* p) {
+    const int* q = reinterpret_cast<const int*>(p);
+    return make_int2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+}
+__device__ inline double2 ld_stream(const double2* p) {
+    const double* q = reinterpret_cast<const double*>(p);
+    return make_double2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+}
+
 template <typename T>
 __device__ inline void lds_add(T* p, T v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -89,11 +111,25 @@ __device__ inline void lds_add(T* p, T v) {
 __device__ inline double grid_mismatch_bus(MisArgs const& a, int i) {
 #pragma clang fp contract(off)
     double cr = 0.0, cim = 0.0;
-    for (int q = a.rp[i]; q < a.rp[i + 1]; ++q) {
-        double2 yv = a.y[q];
-        double2 vv = a.v[a.ci[q]];
-        cr += yv.x * vv.x - yv.y * vv.y;
-        cim += yv.x * vv.y + yv.y * vv.x;
+    int const q1 = a.rp[i + 1];
+    // four row entries per step: their loads are issued together, the sums
+    // keep the row order
+    for (int q = a.rp[i]; q < q1; q += 4) {
+        int c[4];
+        double2 yv[4], vv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c[u] = q + u < q1 ? a.ci[q + u] : i;
+            yv[u] = q + u < q1 ? a.y[q + u] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) vv[u] = a.v[c[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (q + u < q1) {
+                cr += yv[u].x * vv[u].x - yv[u].y * vv[u].y;
+                cim += yv[u].x * vv[u].y + yv[u].y * vv[u].x;
+            }
     }
     double2 vi = a.v[i];
     double sr = vi.x * cr + vi.y * cim;
@@ -202,30 +238,30 @@ __device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArg
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) {
         int const q = tk.pair_beg + k * kWave + lane;
-        pr[k] = q < tk.pair_end ? a.pairs[q] : make_int4(0, 0, 0, 0);
+        pr[k] = q < tk.pair_end ? ld_stream(a.pairs + q) : make_int4(0, 0, 0, 0);
     }
     int2 ex[kExtChunks];
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k) {
         int const q = tk.ext_beg + k * kWave + lane;
-        ex[k] = q < tk.ext_end ? a.ext[q] : make_int2(0, -1);
+        ex[k] = q < tk.ext_end ? ld_stream(a.ext + q) : make_int2(0, -1);
     }
-    int2 const rh = tk.rhs_beg + lane < tk.rhs_end ? a.rhs[tk.rhs_beg + lane] : make_int2(0, -1);
-    int2 const vv = tk.vx_beg + lane < tk.vx_end ? a.vx[tk.vx_beg + lane] : make_int2(0, -1);
+    int2 const rh = tk.rhs_beg + lane < tk.rhs_end ? ld_stream(a.rhs + tk.rhs_beg + lane) : make_int2(0, -1);
+    int2 const vv = tk.vx_beg + lane < tk.vx_end ? ld_stream(a.vx + tk.vx_beg + lane) : make_int2(0, -1);
     // values
     double2 yv[kPairChunks], vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) {
         bool const live = (pr[k].w >> 16) != 0, diag = pr[k].x == pr[k].y;
         double2 const z = make_double2(0.0, 0.0);
-        yv[k] = live ? ma.y[pr[k].z] : z;
+        yv[k] = live ? ld_stream(a.ypair + tk.pair_beg + k * kWave + lane) : z;
         vi[k] = live ? ma.v[pr[k].x] : z;
         vj[k] = live && !diag ? ma.v[pr[k].y] : z;
         si[k] = live && diag ? ma.s[pr[k].x] : z;
     }
     T exv[kExtChunks];
 #pragma unroll
-    for (int k = 0; k < kExtChunks; ++k) exv[k] = ex[k].y >= 0 ? front[ex[k].x] : T(0);
+    for (int k = 0; k < kExtChunks; ++k) exv[k] = ex[k].y >= 0 ? ld_stream(front + ex[k].x) : T(0);
     double const fv = rh.y >= 0 ? ma.f[rh.x] : 0.0;
     T const vxv = vv.y >= 0 ? vec[vv.x] : T(0);
     T* F = pool + gi * rows * ld;
@@ -246,14 +282,14 @@ __device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArg
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) put_pair(pr[k], yv[k], vi[k], vj[k], si[k]);
     for (int q = tk.pair_beg + kPairChunks * kWave + lane; q < tk.pair_end; q += kWave) {
-        int4 const p4 = a.pairs[q];
+        int4 const p4 = ld_stream(a.pairs + q);
         bool const diag = p4.x == p4.y;
         double2 const z = make_double2(0.0, 0.0);
-        put_pair(p4, ma.y[p4.z], ma.v[p4.x], diag ? z : ma.v[p4.y], diag ? ma.s[p4.x] : z);
+        put_pair(p4, ld_stream(a.ypair + q), ma.v[p4.x], diag ? z : ma.v[p4.y], diag ? ma.s[p4.x] : z);
     }
     if (rh.y >= 0) pool[rh.y] = static_cast<T>(-1.0 * fv);
     for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave) {
-        int2 const e = a.rhs[q];
+        int2 const e = ld_stream(a.rhs + q);
         pool[e.y] = static_cast<T>(-1.0 * ma.f[e.x]);
     }
     wave_sync();
@@ -261,17 +297,59 @@ __device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArg
     for (int k = 0; k < kExtChunks; ++k)
         if (ex[k].y >= 0) lds_add(pool + ex[k].y, exv[k]);
     for (int q = tk.ext_beg + kExtChunks * kWave + lane; q < tk.ext_end; q += kWave) {
-        int2 const e = a.ext[q];
-        lds_add(pool + e.y, front[e.x]);
+        int2 const e = ld_stream(a.ext + q);
+        lds_add(pool + e.y, ld_stream(front + e.x));
     }
     if (vv.y >= 0) lds_add(pool + vv.y, vxv);
     for (int q = tk.vx_beg + kWave + lane; q < tk.vx_end; q += kWave) {
-        int2 const e = a.vx[q];
+        int2 const e = ld_stream(a.vx + q);
         lds_add(pool + e.y, vec[e.x]);
     }
     wave_sync();
     int const m = fm.m, p = fm.p;
     int const p_max = wave_group_max<L>(p);
+    if constexpr (MM <= 16) {
+        // dense elimination in registers: lane t holds column t (rows 0..MM-1)
+        // and w[t]; step k broadcasts column k and w[k] from the group's lane k
+        T col[MM];
+#pragma unroll
+        for (int r = 0; r < MM; ++r) col[r] = (t < m && r < m) ? F[r * ld + t] : T(0);
+        T wt = t < m ? w[t] : T(0);
+#pragma unroll
+        for (int k = 0; k < MM; ++k) {
+            if (k >= p_max) continue;  // uniform; keeps the loop fully unrolled
+            int const src = gi * L + k;
+            T lk[MM];
+#pragma unroll
+            for (int i = k; i < MM; ++i) lk[i] = __shfl(col[i], src);
+            T const wk = __shfl(wt, src);
+            bool const act = k < p;
+            T piv = act ? lk[k] : T(1);
+            if (tabs(piv) < T(1e-14)) {
+                if (act && t == 0) fa.fail[fm.inst] = 1;
+                piv = T(1);
+            }
+            T const rp = T(1) / piv;
+            if (act && t > k && t < m) {
+                T const ukj = col[k];
+                T lt = T(0);
+#pragma unroll
+                for (int i = k + 1; i < MM; ++i) {
+                    if (i < m) col[i] -= (lk[i] * rp) * ukj;
+                    if (i == t) lt = lk[i];
+                }
+                wt -= (lt * rp) * wk;
+            }
+        }
+        if (t < m) {
+            T* Gm = front + fm.off;
+#pragma unroll
+            for (int r = 0; r < MM; ++r)
+                if (r < m && t >= (r < p ? r : p)) Gm[r * m + t] = col[r];
+            vec[fm.voff + t] = wt;
+        }
+        return;
+    }
     for (int k = 0; k < p_max; ++k) {
         T piv = k < p ? F[k * ld + k] : T(1);
         bool const tiny = tabs(piv) < T(1e-14);
@@ -307,7 +385,7 @@ __device__ void grid_backward_task(FusedArgs const& a, FrontArgs const& fa, Grid
     int const rows = ld - 1, wbase = G * rows * ld;
     FrontMeta fm{};
     if (gi < (tk.cls_cnt & 0xff)) fm = fa.meta[tk.front_beg + gi];
-    int2 const so = tk.sol_beg + lane < tk.sol_end ? a.sol[tk.sol_beg + lane] : make_int2(0, -1);
+    int2 const so = tk.sol_beg + lane < tk.sol_end ? ld_stream(a.sol + tk.sol_beg + lane) : make_int2(0, -1);
     T const sv = so.y >= 0 ? static_cast<T>(sol[so.x]) : T(0);
     int const m = fm.m, p = fm.p;
     T const wv = t < p ? vec[fm.voff + t] : T(0);
@@ -317,10 +395,40 @@ __device__ void grid_backward_task(FusedArgs const& a, FrontArgs const& fa, Grid
     wave_sync();  // the previous task of this wavefront has finished with the region
     if (so.y >= 0) pool[so.y] = sv;
     for (int q = tk.sol_beg + kWave + lane; q < tk.sol_end; q += kWave) {
-        int2 const e = a.sol[q];
+        int2 const e = ld_stream(a.sol + q);
         pool[e.y] = static_cast<T>(sol[e.x]);
     }
     if (t < p) w[t] = wv;
+    if constexpr (MM <= 16) {
+        // lane t < p holds row t of U; x of the structure slots and the steps'
+        // x_k come from the owning lanes by shuffles
+        T urow[MM];
+        const T* Gr = front + fm.off + static_cast<long long>(t) * m;
+#pragma unroll
+        for (int q = 0; q < MM; ++q) urow[q] = (t < p && q < m) ? ld_stream(Gr + q) : T(0);
+        wave_sync();
+        T acc = t < m ? w[t] : T(0);
+        T xs[MM];
+#pragma unroll
+        for (int q = 0; q < MM; ++q) xs[q] = __shfl(acc, gi * L + q);
+        if (t < p) {
+#pragma unroll
+            for (int q = 0; q < MM; ++q)
+                if (q >= p && q < m) acc -= urow[q] * xs[q];
+        }
+        int const p_max = wave_group_max<L>(p);
+#pragma unroll
+        for (int k = MM - 1; k >= 0; --k) {
+            if (k < p_max) {
+                T const xk = __shfl(acc / urow[k], gi * L + k);
+                if (k < p) {
+                    if (t < k) acc -= urow[k] * xk;
+                    if (t == k) sol[cols[k]] = static_cast<double>(xk);
+                }
+            }
+        }
+        return;
+    }
     load_rows<T, MM, L, CH>(F, ld, front + fm.off, m, p, t, t < m);
     wave_sync();
     if (t < p) {
@@ -366,6 +474,12 @@ __device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArg
     }
 }
 
+// Per-unit Y of every pair in pair order, once per solve.
+__global__ void __launch_bounds__(256) k_gather_ypair(int n, const int4* pairs, const double2* y, double2* ypair) {
+    int const k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) ypair[k] = y[pairs[k].z];
+}
+
 // This wavefront's tasks q, q + NW, ... < qe of one level; the next
 // descriptor is loaded while the current task runs.
 template <typename T, bool BWD, int NW>
@@ -399,7 +513,7 @@ __global__ void __launch_bounds__(NW* kWave) k_nr_grid(FusedArgs a, MisArgs ma, 
     constexpr int NT = NW * kWave;
     extern __shared__ __align__(16) unsigned char grid_smem[];
     __shared__ double red[NW];
-    int const g = a.order[blockIdx.x];
+    int const g = a.order[a.order_off + blockIdx.x];
     int const b0 = a.bus_off[g], b1 = a.bus_off[g + 1];
     int const e0 = a.lvl_beg[g], e1 = a.lvl_beg[g + 1];
     int const tid = threadIdx.x, wave = tid / kWave;

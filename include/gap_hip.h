@@ -136,6 +136,21 @@ int gaphip_plan_set_ybus(gaphip_plan* plan, int inst, const int* row_ptr, const 
 enum { GAPHIP_SCHED_AUTO = 0, GAPHIP_SCHED_LEVELS = 1, GAPHIP_SCHED_GRID = 2 };
 int gaphip_plan_set_schedule(gaphip_plan* plan, int schedule, int* effective);
 
+/* Launch shape of the grid schedule.  A network whose elimination tree holds at
+ * least grid_big_work front slots plus factor entries gets grid_big_waves
+ * wavefronts per workgroup, the others grid_waves (each 1, 2, 4 or 8); the two
+ * groups run concurrently.  grid_big_work < 0 puts every network in the first
+ * group.  Results do not depend on the tuning.  Defaults: 2, 8, -1.
+ * gap_amd.tuning.autotune() times candidates on a plan and sets the fastest.
+ * No counterpart in the reference. */
+typedef struct gaphip_tuning {
+    int grid_waves;
+    int grid_big_waves;
+    long long grid_big_work;
+} gaphip_tuning;
+int gaphip_plan_get_tuning(const gaphip_plan* plan, gaphip_tuning* out);
+int gaphip_plan_set_tuning(gaphip_plan* plan, const gaphip_tuning* tuning);
+
 typedef struct gaphip_solve_opts {
     double tolerance;   /* PowerFlowConfig::tolerance (powerflow_interface.h:14)       */
     int max_iterations; /* PowerFlowConfig::max_iterations (powerflow_interface.h:15)  */

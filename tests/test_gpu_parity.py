@@ -164,6 +164,26 @@ def test_grid_schedule_matches_levels(device, precision):
     assert all(np.array_equal(a, b) for a, b in zip(rg["v"], rg2["v"]))
 
 
+def test_grid_tuning_does_not_change_results(device):
+    from gap_amd import tuning
+    nets = _batch(12, 321) + [gridgen.small_mesh(4000, seed=5, n_extra=40, load_factor=0.5)]
+    p = capi.Plan(nets, precision="mixed", schedule="grid")
+    ref = p.solve(1e-8, 100, 100e6)
+    for gw, bw, work in [(1, 8, 20_000), (4, 4, -1), (8, 2, 0), (2, 8, 40_000)]:
+        p.set_tuning(gw, bw, work)
+        assert p.tuning() == {"grid_waves": gw, "grid_big_waves": bw, "grid_big_work": work}
+        r = p.solve(1e-8, 100, 100e6)
+        assert np.array_equal(r["iterations"], ref["iterations"])
+        assert all(np.array_equal(a, b) for a, b in zip(r["v"], ref["v"]))
+    with pytest.raises(capi.GapHipError):
+        p.set_tuning(3, 8, 0)
+    out = tuning.autotune(p, candidates={"grid_waves": (1, 2), "grid_big_waves": (8,), "grid_big_work": (-1, 40_000)},
+                          repeats=1)
+    assert p.tuning() == out["tuning"] and len(out["table"]) == 4
+    r = p.solve(1e-8, 100, 100e6)
+    assert all(np.array_equal(a, b) for a, b in zip(r["v"], ref["v"]))
+
+
 def test_grid_schedule_vs_oracle(device, oracle):
     nets = _batch(6, 900)
     r = capi.Plan(nets, schedule="grid").solve(1e-8, 100, 100e6)
