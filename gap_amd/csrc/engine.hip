@@ -672,19 +672,42 @@ void pack_fronts(std::vector<LUPlan const*> const& plans, std::vector<long long>
 // non-empty (level, size class).  `hook(kid, bytes, fn)` wraps every launch.
 template <typename T>
 struct FrontRunner {
-    static constexpr int kPanel = 32;
+    static constexpr int kPanel = kBigPanel;
     // blocked partial LU of the fronts [beg, beg+nb) of one (level, class > 128) group
     static void big_factor(DevFronts const& d, FrontArgs const& a, int beg, int nb, int mmax, int pmax,
                            T const* jv, T* front, hipStream_t s) {
         hipLaunchKernelGGL(k_big_assemble<T>, dim3(mmax, nb), dim3(256), 0, s, a, d.big(), beg, jv, front);
         for (int k0 = 0; k0 < pmax; k0 += kPanel) {
-            hipLaunchKernelGGL(k_big_panel<T>, dim3(nb), dim3(256), 0, s, a, beg, k0, kPanel, front);
+            hipLaunchKernelGGL(k_big_diag<T>, dim3(nb), dim3(256), 0, s, a, beg, k0, front);
             int const rest = mmax - std::min(k0 + kPanel, pmax);
             if (rest <= 0) continue;
+            hipLaunchKernelGGL(k_big_trsm_rows<T>, dim3(blocks_for(rest, 256), nb), dim3(256), 0, s, a, beg, k0, front);
             hipLaunchKernelGGL(k_big_trsm<T>, dim3(blocks_for(rest, 256), nb), dim3(256), 0, s, a, beg, k0, kPanel,
                                front);
             int const tiles = blocks_for(rest, 64);
             hipLaunchKernelGGL(k_big_gemm<T>, dim3(tiles, tiles, nb), dim3(256), 0, s, a, beg, k0, kPanel, front);
+        }
+    }
+    // blocked forward / backward solves of the fronts [beg, beg+nb) of a large-front group
+    static void big_forward(DevFronts const&, FrontArgs const& a, int beg, int nb, int mmax, int pmax, T const* front,
+                            T* vec, double const* rhs, double sign, hipStream_t s) {
+        hipLaunchKernelGGL(k_big_fwd_init<T>, dim3(nb), dim3(256), 0, s, a, beg, vec, rhs, sign);
+        for (int k0 = 0; k0 < pmax; k0 += kSolvePanel) {
+            hipLaunchKernelGGL(k_big_fwd_diag<T>, dim3(nb), dim3(kWave), 0, s, a, beg, k0, front, vec);
+            int const rest = mmax - std::min(k0 + kSolvePanel, pmax);
+            if (rest > 0)
+                hipLaunchKernelGGL(k_big_fwd_update<T>, dim3(blocks_for(rest, 256), nb), dim3(256), 0, s, a, beg, k0,
+                                   front, vec);
+        }
+    }
+    static void big_backward(DevFronts const&, FrontArgs const& a, int beg, int nb, int pmax, T const* front, T* vec,
+                             double* sol, hipStream_t s) {
+        hipLaunchKernelGGL(k_big_bwd_init<T>, dim3(blocks_for(pmax, 256), nb), dim3(256), 0, s, a, beg, front, vec, sol);
+        for (int k0 = ((pmax - 1) / kSolvePanel) * kSolvePanel; k0 >= 0; k0 -= kSolvePanel) {
+            hipLaunchKernelGGL(k_big_bwd_diag<T>, dim3(nb), dim3(kWave), 0, s, a, beg, k0, front, vec, sol);
+            if (k0 > 0)
+                hipLaunchKernelGGL(k_big_bwd_update<T>, dim3(blocks_for(k0, 256), nb), dim3(256), 0, s, a, beg, k0,
+                                   front, vec);
         }
     }
     template <typename Hook>
@@ -730,7 +753,12 @@ struct FrontRunner {
                         case 3: hipLaunchKernelGGL((k_front_fwd_lds<T, 32, 32, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
                         case 4: hipLaunchKernelGGL((k_front_fwd_lds<T, 64, 64, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, rhs, sign); break;
                         case 5: hipLaunchKernelGGL((k_front_fwd_lds<T, 128, 128, 128, TR>), dim3(gb), dim3(128), 0, s, a, beg, nb, front, vec, rhs, sign); break;
-                        default: hipLaunchKernelGGL((k_front_fwd<T, TR>), dim3(nb), dim3(256), 0, s, a, beg, front, vec, rhs, sign);
+                        default:
+                            if constexpr (TR) {
+                                hipLaunchKernelGGL((k_front_fwd<T, TR>), dim3(nb), dim3(256), 0, s, a, beg, front, vec, rhs, sign);
+                            } else {
+                                big_forward(d, a, beg, nb, d.cls_mmax[b], d.cls_pmax[b], front, vec, rhs, sign, s);
+                            }
                     }
                 });
             }
@@ -754,7 +782,12 @@ struct FrontRunner {
                         case 3: hipLaunchKernelGGL((k_front_bwd_lds<T, 32, 32, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
                         case 4: hipLaunchKernelGGL((k_front_bwd_lds<T, 64, 64, 64, TR>), dim3(gb), dim3(64), 0, s, a, beg, nb, front, vec, sol); break;
                         case 5: hipLaunchKernelGGL((k_front_bwd_lds<T, 128, 128, 128, TR>), dim3(gb), dim3(128), 0, s, a, beg, nb, front, vec, sol); break;
-                        default: hipLaunchKernelGGL((k_front_bwd<T, TR>), dim3(nb), dim3(256), 0, s, a, beg, front, vec, sol);
+                        default:
+                            if constexpr (TR) {
+                                hipLaunchKernelGGL((k_front_bwd<T, TR>), dim3(nb), dim3(256), 0, s, a, beg, front, vec, sol);
+                            } else {
+                                big_backward(d, a, beg, nb, d.cls_pmax[b], front, vec, sol, s);
+                            }
                     }
                 });
             }

@@ -461,31 +461,83 @@ __global__ void __launch_bounds__(256) k_big_assemble(FrontArgs a, BigMaps b, in
     }
 }
 
+// Panels of kBigPanel pivots.  For each: the diagonal block A11 is factored by
+// one workgroup per front (multipliers stored below its diagonal), the rows
+// below become L21 = A21 U11^-1 (k_big_trsm_rows), the columns to the right
+// U12 = L11^-1 A12 (k_big_trsm), and the trailing block takes the MFMA update
+// (k_big_gemm).  Per entry the arithmetic is that of right-looking
+// elimination: a_ik -= l_iq u_qk for q = k0 .. k-1 in order, then l_ik = a_ik / u_kk.
+constexpr int kBigPanel = 32;
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_big_panel(FrontArgs a, int begin, int k0, int nb, T* front) {
+__device__ inline T pivot_recip(T piv, int* fail, int inst, bool flag) {
+    if (tabs(piv) < T(1e-14)) {
+        if (flag && fail) fail[inst] = 1;
+        piv = T(1);
+    }
+    return T(1) / piv;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_diag(FrontArgs a, int begin, int k0, T* front) {
+    __shared__ T D[kBigPanel][kBigPanel + 1];
     FrontMeta const fm = a.meta[begin + blockIdx.x];
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
     if (k0 >= p) return;
-    int const kend = min(k0 + nb, p);
+    int const kb = min(kBigPanel, p - k0);
     int const tid = threadIdx.x;
     T* F = front + fm.off;
-    for (int k = k0; k < kend; ++k) {
-        T piv = F[static_cast<long long>(k) * m + k];
-        if (tabs(piv) < T(1e-14)) {
-            if (tid == 0 && a.fail) a.fail[fm.inst] = 1;
-            piv = T(1);
-        }
-        T const rp = T(1) / piv;
-        const T* uk = F + static_cast<long long>(k) * m;
-        for (int i = k + 1 + tid; i < m; i += 256) {
-            T* ri = F + static_cast<long long>(i) * m;
-            T const l = ri[k] * rp;
-            ri[k] = l;
-            for (int j = k + 1; j < kend; ++j) ri[j] -= l * uk[j];
+    for (int e = tid; e < kb * kb; e += 256) D[e / kb][e % kb] = F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb];
+    __syncthreads();
+    for (int k = 0; k < kb; ++k) {
+        T const rp = pivot_recip(D[k][k], a.fail, fm.inst, tid == 0);
+        for (int i = k + 1 + tid; i < kb; i += 256) {
+            T const l = D[i][k] * rp;
+            D[i][k] = l;
+            for (int j = k + 1; j < kb; ++j) D[i][j] -= l * D[k][j];
         }
         __syncthreads();
     }
+    for (int e = tid; e < kb * kb; e += 256) F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb] = D[e / kb][e % kb];
+}
+
+// L21 = A21 U11^-1 for the rows below the panel: a thread per row, the row's
+// panel segment in registers, U11 in LDS.
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_trsm_rows(FrontArgs a, int begin, int k0, T* front) {
+    __shared__ T U[kBigPanel][kBigPanel + 1];
+    __shared__ T R[kBigPanel];
+    FrontMeta const fm = a.meta[begin + blockIdx.y];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
+    if (k0 >= p) return;
+    int const kb = min(kBigPanel, p - k0), kend = k0 + kb;
+    int const tid = threadIdx.x;
+    T* F = front + fm.off;
+    if (kend + static_cast<int>(blockIdx.x) * 256 >= m) return;  // uniform: no rows for this block
+    for (int e = tid; e < kb * kb; e += 256) U[e / kb][e % kb] = F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb];
+    __syncthreads();
+    if (tid < kb) R[tid] = pivot_recip(U[tid][tid], nullptr, 0, false);
+    __syncthreads();
+    int const i = kend + blockIdx.x * 256 + tid;
+    if (i >= m) return;
+    T* ri = F + static_cast<long long>(i) * m + k0;
+    T x[kBigPanel];
+#pragma unroll
+    for (int k = 0; k < kBigPanel; ++k) x[k] = k < kb ? ri[k] : T(0);
+#pragma unroll
+    for (int k = 0; k < kBigPanel; ++k) {
+        if (k < kb) {
+            T acc = x[k];
+#pragma unroll
+            for (int q = 0; q < k; ++q) acc -= x[q] * U[q][k];
+            x[k] = acc * R[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kBigPanel; ++k)
+        if (k < kb) ri[k] = x[k];
 }
 
 template <typename T>
@@ -506,13 +558,33 @@ __global__ void __launch_bounds__(256) k_big_trsm(FrontArgs a, int begin, int k0
     }
 }
 
-// C[kend:m, kend:m] -= L[kend:m, k0:kend] * U[k0:kend, kend:m]; 64x64 tiles,
-// 256 threads with a 4x4 register tile each, the K slice (<= 32) staged in LDS.
+// MFMA 16x16x4 blocks.  Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16];
+// accumulator register r holds row 4 (l / 16) + r (f32) or l / 16 + 4 r (f64) of
+// column l % 16.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+    using V = f32x4;
+    __device__ static V mma(float x, float y, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <>
+struct Mfma<double> {
+    using V = f64x4;
+    __device__ static V mma(double x, double y, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0); }
+    __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+// C[kend:m, kend:m] -= L21 * U12 (K = the panel, <= 32): 64x64 tiles, four
+// wavefronts with 2x2 MFMA blocks each, A and B staged k-major in LDS.
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0, int nb, T* front) {
-    constexpr int TB = 64, KB = 32;
-    __shared__ T As[KB][TB + 1];
-    __shared__ T Bs[KB][TB + 1];
+    constexpr int TB = 64, KB = kBigPanel;
+    __shared__ T As[KB][TB + 4];
+    __shared__ T Bs[KB][TB + 4];
     FrontMeta const fm = a.meta[begin + blockIdx.z];
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
@@ -524,30 +596,41 @@ __global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0
     T* F = front + fm.off;
     int const tid = threadIdx.x;
     for (int e = tid; e < KB * TB; e += 256) {
-        int const kk = e / TB, x = e % TB;
+        int const x = e % TB, kk = e / TB;
         int const gi = i0 + x, gj = j0 + x;
         As[kk][x] = (kk < kb && gi < m) ? F[static_cast<long long>(gi) * m + k0 + kk] : T(0);
         Bs[kk][x] = (kk < kb && gj < m) ? F[static_cast<long long>(k0 + kk) * m + gj] : T(0);
     }
     __syncthreads();
-    int const tr = (tid / 16) * 4, tc = (tid % 16) * 4;
-    T acc[4][4] = {};
-    for (int kk = 0; kk < kb; ++kk) {
-        T av[4], bv[4];
-        for (int u = 0; u < 4; ++u) av[u] = As[kk][tr + u];
-        for (int u = 0; u < 4; ++u) bv[u] = Bs[kk][tc + u];
-        for (int u = 0; u < 4; ++u)
-            for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+    using V = typename Mfma<T>::V;
+    int const wave = tid / kWave, lane = tid % kWave;
+    int const wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    V acc[2][2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc[u][v] = V{};
+    for (int kk = 0; kk < kb; kk += 4) {
+        int const k = kk + (lane >> 4);
+        T av[2], bv[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) av[u] = As[k][wr + u * 16 + (lane & 15)];
+#pragma unroll
+        for (int v = 0; v < 2; ++v) bv[v] = Bs[k][wc + v * 16 + (lane & 15)];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) acc[u][v] = Mfma<T>::mma(av[u], bv[v], acc[u][v]);
     }
-    for (int u = 0; u < 4; ++u) {
-        int const gi = i0 + tr + u;
-        if (gi >= m) continue;
-        T* ci = F + static_cast<long long>(gi) * m;
-        for (int v = 0; v < 4; ++v) {
-            int const gj = j0 + tc + v;
-            if (gj < m) ci[gj] -= acc[u][v];
-        }
-    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int const gi = i0 + wr + u * 16 + Mfma<T>::row(lane, r), gj = j0 + wc + v * 16 + (lane & 15);
+                if (gi < m && gj < m) F[static_cast<long long>(gi) * m + gj] -= acc[u][v][r];
+            }
 }
 
 // Forward solve L y = b (TR: U^T z = b), front by front from the leaves (large
@@ -620,6 +703,151 @@ __global__ void __launch_bounds__(256) k_front_bwd(FrontArgs a, int begin, const
         if (tid == 0) sol[cols[k]] = static_cast<double>(xk);
         __syncthreads();
     }
+}
+
+// Blocked triangular solves of large fronts (the Newton solves; the transposed
+// pair keeps the kernels above).  Panels of kSolvePanel pivots: the diagonal
+// block by one wavefront per front (lane t owns row k0 + t, the step values
+// broadcast by shuffles), the rest of the rows by many workgroups.  Every row
+// subtracts its terms one by one in the same order as the kernels above
+// (ascending k forward, descending k backward).
+constexpr int kSolvePanel = 64;
+
+// w = vec: -F of the pivot slots (rhs_sign), plus the children's update vectors
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_fwd_init(FrontArgs a, int begin, T* vec, const double* rhs,
+                                                      double rhs_sign) {
+    FrontMeta const fm = a.meta[begin + blockIdx.x];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
+    int const tid = threadIdx.x;
+    T* w = vec + fm.voff;
+    const int* rows = a.slot_row + fm.slot_beg;
+    for (int t = tid; t < m; t += 256) w[t] = t < p ? static_cast<T>(rhs_sign * rhs[rows[t]]) : T(0);
+    __syncthreads();
+    for (int cq = fm.child_beg; cq < fm.child_end; ++cq) {
+        ChildInfo const c = a.cinfo[cq];
+        int const nu = c.mc - c.pc;
+        const T* wc = vec + c.voff + c.pc;
+        const int* rl = a.rel + c.rel_off;
+        for (int r = tid; r < nu; r += 256) w[rl[r]] += wc[r];
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_big_fwd_diag(FrontArgs a, int begin, int k0, const T* front, T* vec) {
+    FrontMeta const fm = a.meta[begin + blockIdx.x];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
+    if (k0 >= p) return;
+    int const kb = min(kSolvePanel, p - k0), t = threadIdx.x;
+    T* w = vec + fm.voff + k0;
+    const T* row = front + fm.off + static_cast<long long>(k0 + t) * m + k0;
+    T lr[kSolvePanel];
+#pragma unroll
+    for (int k = 0; k < kSolvePanel; ++k) lr[k] = (k < t && t < kb) ? row[k] : T(0);
+    T wt = t < kb ? w[t] : T(0);
+#pragma unroll
+    for (int k = 0; k < kSolvePanel; ++k) {
+        if (k < kb) {
+            T const yk = __shfl(wt, k);
+            if (t > k && t < kb) wt -= lr[k] * yk;
+        }
+    }
+    if (t < kb) w[t] = wt;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_fwd_update(FrontArgs a, int begin, int k0, const T* front, T* vec) {
+    __shared__ T y[kSolvePanel];
+    FrontMeta const fm = a.meta[begin + blockIdx.y];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
+    if (k0 >= p) return;
+    int const kb = min(kSolvePanel, p - k0), kend = k0 + kb;
+    if (kend + static_cast<int>(blockIdx.x) * 256 >= m) return;  // uniform
+    T* w = vec + fm.voff;
+    if (threadIdx.x < kb) y[threadIdx.x] = w[k0 + threadIdx.x];
+    __syncthreads();
+    int const i = kend + blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const T* ri = front + fm.off + static_cast<long long>(i) * m + k0;
+    T acc = w[i];
+    for (int k = 0; k < kb; ++k) acc -= ri[k] * y[k];
+    w[i] = acc;
+}
+
+// pivot rows: w_k -= U[k][t] x_t over the structure slots t = p .. m-1
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_bwd_init(FrontArgs a, int begin, const T* front, T* vec,
+                                                      const double* sol) {
+    FrontMeta const fm = a.meta[begin + blockIdx.y];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
+    int const k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= p || p == m) return;
+    const int* cols = a.slot_col + fm.slot_beg;
+    const T* rk = front + fm.off + static_cast<long long>(k) * m;
+    T* w = vec + fm.voff;
+    T acc = w[k];
+    for (int t = p; t < m; ++t) acc -= rk[t] * static_cast<T>(sol[cols[t]]);
+    w[k] = acc;
+}
+
+// panel [k0, kend) from its last row: x_k = w_k / U[k][k], written to the
+// solution and over w_k
+template <typename T>
+__global__ void __launch_bounds__(64) k_big_bwd_diag(FrontArgs a, int begin, int k0, const T* front, T* vec,
+                                                     double* sol) {
+    FrontMeta const fm = a.meta[begin + blockIdx.x];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
+    if (k0 >= p) return;
+    int const kb = min(kSolvePanel, p - k0), t = threadIdx.x;
+    const int* cols = a.slot_col + fm.slot_beg + k0;
+    T* w = vec + fm.voff + k0;
+    const T* row = front + fm.off + static_cast<long long>(k0 + t) * m + k0;
+    T ur[kSolvePanel];
+#pragma unroll
+    for (int k = 0; k < kSolvePanel; ++k) ur[k] = (k >= t && t < kb && k < kb) ? row[k] : T(0);
+    T wt = t < kb ? w[t] : T(0);
+    T diag = T(1);
+#pragma unroll
+    for (int k = 0; k < kSolvePanel; ++k)
+        if (k == t) diag = ur[k];
+#pragma unroll
+    for (int k = kSolvePanel - 1; k >= 0; --k) {
+        if (k < kb) {
+            T const xk = __shfl(wt / diag, k);
+            if (t < k) wt -= ur[k] * xk;
+            if (t == k) {
+                sol[cols[k]] = static_cast<double>(xk);
+                w[k] = xk;
+            }
+        }
+    }
+}
+
+// rows above the panel: w_i -= U[i][k] x_k for k = kend-1 .. k0
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_bwd_update(FrontArgs a, int begin, int k0, const T* front, T* vec) {
+    __shared__ T x[kSolvePanel];
+    FrontMeta const fm = a.meta[begin + blockIdx.y];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
+    if (k0 >= p) return;
+    int const kb = min(kSolvePanel, p - k0);
+    if (static_cast<int>(blockIdx.x) * 256 >= k0) return;  // uniform
+    T* w = vec + fm.voff;
+    if (threadIdx.x < kb) x[threadIdx.x] = w[k0 + threadIdx.x];
+    __syncthreads();
+    int const i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= k0) return;
+    const T* ri = front + fm.off + static_cast<long long>(i) * m + k0;
+    T acc = w[i];
+    for (int k = kb - 1; k >= 0; --k) acc -= ri[k] * x[k];
+    w[i] = acc;
 }
 
 // Small-front forward / backward solves, G fronts per workgroup as above

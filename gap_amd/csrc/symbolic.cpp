@@ -204,7 +204,11 @@ bool plan_nr(Net const& net, YPlan const& y, NRPlan& p, std::string& err) {
 
 // --------------------------------------------------------- minimum degree
 // Explicit elimination graph with a lazy (degree, node) heap; ties go to the
-// lower node index so the order is deterministic.
+// lower node index so the order is deterministic.  A remaining graph of at
+// least kDenseMin nodes whose minimum degree reaches half its size is ordered
+// directly (it becomes one dense front either way).
+constexpr int kDenseMin = 512;
+
 std::vector<int> min_degree_order(int n, std::vector<int> const& adj_ptr, std::vector<int> const& adj) {
     std::vector<std::vector<int>> g(n);
     for (int i = 0; i < n; ++i) g[i].assign(adj.begin() + adj_ptr[i], adj.begin() + adj_ptr[i + 1]);
@@ -214,12 +218,32 @@ std::vector<int> min_degree_order(int n, std::vector<int> const& adj_ptr, std::v
     using Ent = std::pair<int, int>;
     std::priority_queue<Ent, std::vector<Ent>, std::greater<Ent>> heap;
     for (int i = 0; i < n; ++i) heap.push({static_cast<int>(g[i].size()), i});
-    int stamp = 0;
+    int stamp = 0, left = n;
     while (!heap.empty()) {
         auto [d, v] = heap.top();
         heap.pop();
         if (gone[v] || d != static_cast<int>(g[v].size())) continue;
+        if (d == left - 1 && left > 2) {
+            // the remaining graph is a clique: every node has degree left - 1,
+            // and eliminating one keeps the rest a clique with equal degrees,
+            // so the order continues by ascending index (the tie rule)
+            for (int u = 0; u < n; ++u)
+                if (!gone[u]) order.push_back(u);
+            break;
+        }
+        if (left >= kDenseMin && 2LL * d >= left - 1) {
+            // at least half dense: eliminating it fills it in after a few
+            // steps whatever the order, while each step here would cost
+            // O(degree^2); finish by (current degree, index)
+            std::vector<std::pair<int, int>> rest;
+            for (int u = 0; u < n; ++u)
+                if (!gone[u]) rest.push_back({static_cast<int>(g[u].size()), u});
+            std::sort(rest.begin(), rest.end());
+            for (auto const& e : rest) order.push_back(e.second);
+            break;
+        }
         gone[v] = 1;
+        --left;
         order.push_back(v);
         std::vector<int> nb;
         nb.swap(g[v]);

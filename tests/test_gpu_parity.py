@@ -120,6 +120,25 @@ def test_large_grid_fixed_point(device, oracle):
     assert np.abs(f).max() < 1e-8
 
 
+def test_100k_grid_mixed_vs_fp64(device, oracle):
+    """100k-bus meshed grid (root front above 15k slots): FP32 factors with FP64
+    residuals reach the same solution as the FP64 path, and both are fixed
+    points of the oracle's own mismatch."""
+    net = gridgen.c2_grid()
+    a = capi.Plan([net], precision="fp64").solve(1e-8, 100, 100e6)
+    b = capi.Plan([net], precision="mixed").solve(1e-8, 100, 100e6)
+    assert a["converged"][0] and b["converged"][0]
+    dv = np.abs(a["v"][0] - b["v"][0]).max()
+    print("100k grid: iterations fp64", int(a["iterations"][0]), "mixed", int(b["iterations"][0]), "max|dV|", dv)
+    assert dv <= V_TOL
+    assert abs(int(a["iterations"][0]) - int(b["iterations"][0])) <= 1
+    y = oracle.build_ybus(net)
+    ypu = (y[0], y[1], y[2] * (net.u_rated[0] ** 2 / 100e6))
+    for r in (a, b):
+        f = oracle.mismatches(net, r["v"][0], ypu) / 100e6
+        assert np.abs(f).max() < 1e-8
+
+
 @pytest.mark.parametrize("schedule", ["auto", "levels"])
 def test_mixed_precision_close_to_fp64(device, schedule):
     nets = [gridgen.small_mesh(n, seed=s) for n, s in [(200, 0), (800, 4), (3000, 9)]]
