@@ -684,7 +684,7 @@ struct FrontRunner {
             hipLaunchKernelGGL(k_big_trsm_rows<T>, dim3(blocks_for(rest, 256), nb), dim3(256), 0, s, a, beg, k0, front);
             hipLaunchKernelGGL(k_big_trsm<T>, dim3(blocks_for(rest, 256), nb), dim3(256), 0, s, a, beg, k0, kPanel,
                                front);
-            int const tiles = blocks_for(rest, 64);
+            int const tiles = blocks_for(rest, 128);
             hipLaunchKernelGGL(k_big_gemm<T>, dim3(tiles, tiles, nb), dim3(256), 0, s, a, beg, k0, kPanel, front);
         }
     }

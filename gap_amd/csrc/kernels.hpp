@@ -467,7 +467,7 @@ __global__ void __launch_bounds__(256) k_big_assemble(FrontArgs a, BigMaps b, in
 // U12 = L11^-1 A12 (k_big_trsm), and the trailing block takes the MFMA update
 // (k_big_gemm).  Per entry the arithmetic is that of right-looking
 // elimination: a_ik -= l_iq u_qk for q = k0 .. k-1 in order, then l_ik = a_ik / u_kk.
-constexpr int kBigPanel = 32;
+constexpr int kBigPanel = 64;
 
 template <typename T>
 __device__ inline T pivot_recip(T piv, int* fail, int inst, bool flag) {
@@ -540,22 +540,39 @@ __global__ void __launch_bounds__(256) k_big_trsm_rows(FrontArgs a, int begin, i
         if (k < kb) ri[k] = x[k];
 }
 
+// U12 = L11^-1 A12 for the columns right of the panel: a thread per column,
+// the column's panel segment in registers, L11 in LDS.
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_trsm(FrontArgs a, int begin, int k0, int nb, T* front) {
+    __shared__ T Ls[kBigPanel][kBigPanel + 1];
     FrontMeta const fm = a.meta[begin + blockIdx.y];
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
     if (k0 >= p) return;
-    int const kend = min(k0 + nb, p);
+    int const kend = min(k0 + nb, p), kb = kend - k0;
+    if (kend + static_cast<int>(blockIdx.x) * 256 >= m) return;  // uniform: no columns for this block
+    T* F = front + fm.off;
+    for (int e = threadIdx.x; e < kb * kb; e += 256)
+        Ls[e / kb][e % kb] = F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb];
+    __syncthreads();
     int const j = kend + blockIdx.x * 256 + threadIdx.x;
     if (j >= m) return;
-    T* F = front + fm.off;
-    for (int r = k0 + 1; r < kend; ++r) {
-        T acc = F[static_cast<long long>(r) * m + j];
-        const T* lr = F + static_cast<long long>(r) * m;
-        for (int q = k0; q < r; ++q) acc -= lr[q] * F[static_cast<long long>(q) * m + j];
-        F[static_cast<long long>(r) * m + j] = acc;
+    T* cj = F + static_cast<long long>(k0) * m + j;
+    T x[kBigPanel];
+#pragma unroll
+    for (int r = 0; r < kBigPanel; ++r) x[r] = r < kb ? cj[static_cast<long long>(r) * m] : T(0);
+#pragma unroll
+    for (int r = 1; r < kBigPanel; ++r) {
+        if (r < kb) {
+            T acc = x[r];
+#pragma unroll
+            for (int q = 0; q < r; ++q) acc -= Ls[r][q] * x[q];
+            x[r] = acc;
+        }
     }
+#pragma unroll
+    for (int r = 1; r < kBigPanel; ++r)
+        if (r < kb) cj[static_cast<long long>(r) * m] = x[r];
 }
 
 // MFMA 16x16x4 blocks.  Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16];
@@ -578,13 +595,14 @@ struct Mfma<double> {
     __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
-// C[kend:m, kend:m] -= L21 * U12 (K = the panel, <= 32): 64x64 tiles, four
-// wavefronts with 2x2 MFMA blocks each, A and B staged k-major in LDS.
+// C[kend:m, kend:m] -= L21 * U12 (K = the panel): 128x128 tiles, four
+// wavefronts with 4x4 MFMA blocks each; A and B staged k-major in LDS in K
+// chunks (A read along its rows, so both loads are coalesced).
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0, int nb, T* front) {
-    constexpr int TB = 64, KB = kBigPanel;
-    __shared__ T As[KB][TB + 4];
-    __shared__ T Bs[KB][TB + 4];
+    constexpr int TB = 128, KC = sizeof(T) == 4 ? 32 : 16;
+    __shared__ T As[KC][TB + 4];
+    __shared__ T Bs[KC][TB + 4];
     FrontMeta const fm = a.meta[begin + blockIdx.z];
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
@@ -595,41 +613,54 @@ __global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0
     if (i0 >= m || j0 >= m) return;
     T* F = front + fm.off;
     int const tid = threadIdx.x;
-    for (int e = tid; e < KB * TB; e += 256) {
-        int const x = e % TB, kk = e / TB;
-        int const gi = i0 + x, gj = j0 + x;
-        As[kk][x] = (kk < kb && gi < m) ? F[static_cast<long long>(gi) * m + k0 + kk] : T(0);
-        Bs[kk][x] = (kk < kb && gj < m) ? F[static_cast<long long>(k0 + kk) * m + gj] : T(0);
-    }
-    __syncthreads();
     using V = typename Mfma<T>::V;
     int const wave = tid / kWave, lane = tid % kWave;
-    int const wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    V acc[2][2];
+    int const wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+    // the accumulators start from C and A enters negated, so the MFMAs leave
+    // C - A B; the C loads overlap the K loop
+    V acc[4][4];
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int v = 0; v < 2; ++v) acc[u][v] = V{};
-    for (int kk = 0; kk < kb; kk += 4) {
-        int const k = kk + (lane >> 4);
-        T av[2], bv[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) av[u] = As[k][wr + u * 16 + (lane & 15)];
-#pragma unroll
-        for (int v = 0; v < 2; ++v) bv[v] = Bs[k][wc + v * 16 + (lane & 15)];
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int v = 0; v < 2; ++v) acc[u][v] = Mfma<T>::mma(av[u], bv[v], acc[u][v]);
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int v = 0; v < 2; ++v)
+        for (int v = 0; v < 4; ++v)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 int const gi = i0 + wr + u * 16 + Mfma<T>::row(lane, r), gj = j0 + wc + v * 16 + (lane & 15);
-                if (gi < m && gj < m) F[static_cast<long long>(gi) * m + gj] -= acc[u][v][r];
+                acc[u][v][r] = (gi < m && gj < m) ? F[static_cast<long long>(gi) * m + gj] : T(0);
+            }
+    for (int c0 = 0; c0 < kb; c0 += KC) {
+        int const kc = min(KC, kb - c0);
+        __syncthreads();  // the previous chunk has been consumed
+        for (int e = tid; e < KC * TB; e += 256) {
+            int const ar = e / KC, ak = e % KC;  // A: along the row
+            int const gi = i0 + ar;
+            As[ak][ar] = (ak < kc && gi < m) ? -F[static_cast<long long>(gi) * m + k0 + c0 + ak] : T(0);
+            int const bx = e % TB, bk = e / TB;  // B: along the row of U12
+            int const gj = j0 + bx;
+            Bs[bk][bx] = (bk < kc && gj < m) ? F[static_cast<long long>(k0 + c0 + bk) * m + gj] : T(0);
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kc; kk += 4) {
+            int const k = kk + (lane >> 4);
+            T av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) av[u] = As[k][wr + u * 16 + (lane & 15)];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) bv[v] = Bs[k][wc + v * 16 + (lane & 15)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = Mfma<T>::mma(av[u], bv[v], acc[u][v]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int const gi = i0 + wr + u * 16 + Mfma<T>::row(lane, r), gj = j0 + wc + v * 16 + (lane & 15);
+                if (gi < m && gj < m) F[static_cast<long long>(gi) * m + gj] = acc[u][v][r];
             }
 }
 
