@@ -13,8 +13,10 @@ Also reported:
     the batched/plain ratio;
   * roofline of the dominant kernel family: algorithmic bytes per launch over
     its mean launch time (HIP events around every launch, on the plan's stream);
-  * cpu_baseline: the FP64 C oracle (pforacle.c, 1 thread) on a stratified
-    sample of the same grids.
+  * cpu_baseline: the reference's own CPU Newton-Raphson (compiled from its
+    sources, 1 thread) on the smallest grids of the batch, about 15 s of work;
+    cpu_port_baseline: the FP64 C restatement (pforacle.c, 1 thread) on a
+    stratified sample of the batch.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grids 4096]
 """
@@ -54,6 +56,8 @@ def main():
     ap.add_argument("--grids", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--cpu-sample", type=int, default=1536)
+    ap.add_argument("--cpu-budget", type=float, default=15.0,
+                    help="seconds of reference CPU solves for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe-copy", action="store_true",
                     help="after the run, copy a 512 MiB device tensor 3 times (calibrates HBM byte counters)")
@@ -157,14 +161,15 @@ def main():
     bytes_per_launch = f["bytes"] / max(f["launches"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": fam, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(fam),
             "avg_launch_us": round(avg_ms * 1e3, 3), "launches_per_step": f["launches"],
             "algorithmic_bytes_per_launch": round(bytes_per_launch),
             "kernels_ms_per_step": {k: round(v["ms"], 3) for k, v in rep.items() if v["launches"]}}
 
-    cpu = None
+    cpu, port = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(nets, sizes, args.cpu_sample, tol, max_it, base)
+        cpu = cpu_reference(nets, sizes, args.cpu_budget, tol, max_it, base)
+        port = cpu_port(nets, sizes, args.cpu_sample, tol, max_it, base)
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "solves/s", "n_gpus": 1,
@@ -185,6 +190,7 @@ def main():
         "mixed_vs_fp64_max_abs_dv": dv, "mixed_vs_fp64_max_iteration_diff": it_diff,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_port_baseline": port,
     }
     print(json.dumps(out), flush=True)
     if args.probe_copy:
@@ -197,8 +203,9 @@ def main():
         log(f"[bench] copy probe: 3 x {4 * n} bytes read and written")
 
 
-def cpu_baseline(nets, sizes, count, tol, max_it, base):
-    """FP64 C oracle, one thread, on a stratified sample of the batch."""
+def cpu_port(nets, sizes, count, tol, max_it, base):
+    """FP64 C restatement of the reference path (oracle), one thread, on a
+    stratified sample of the batch."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle
     if not pyoracle.ORACLE_SO.exists():
@@ -212,6 +219,44 @@ def cpu_baseline(nets, sizes, count, tol, max_it, base):
     return {"value": round(len(idx) / dt, 3), "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": f"{len(idx)} grids stratified by size from the same batch "
                       f"({int(sizes[idx].sum())} buses), FP64 C oracle (pforacle.c), {dt:.1f} s"}
+
+
+def cpu_reference(nets, sizes, budget_s, tol, max_it, base):
+    """The reference's own CPU Newton-Raphson (its sources compiled under
+    oracle/_ref), one thread, on the smallest grids of the batch until about
+    budget_s seconds are spent.  Its dense Jacobian and unordered LU grow
+    steeply with size (about 3.7 s for 200 buses), so only the small end of the
+    batch is timed; the figure is an upper bound for the whole batch."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle
+    if not pyoracle.reference_available():
+        return None
+    ref = pyoracle.Reference()
+    order = np.argsort(sizes, kind="stable")
+    done, t0 = [], time.perf_counter()
+    for i in order:
+        ref.solve(nets[i], tol=tol, max_it=max_it, base_power=base, j_call=-1, traj=False)
+        done.append(int(i))
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    sz = sizes[done]
+    return {"value": round(len(done) / dt, 3), "unit": "solves/s", "cores": 1, "kind": "reference",
+            "sample": f"the {len(done)} smallest grids of the batch ({int(sz.min())}-{int(sz.max())} buses, "
+                      f"{int(sz.sum())} buses), reference CPU NR compiled from its sources, {dt:.1f} s; "
+                      f"an upper bound for the batch (the batch's grids reach 5000 buses)"}
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE passes, calibrated), or None."""
+    p = ROOT / "profiles" / "batch4096_traffic.json"
+    if not p.exists():
+        return None
+    d = json.loads(p.read_text())
+    if d.get("kernel") != kernel:
+        return None
+    return d.get("hbm_bytes_per_launch")
 
 
 if __name__ == "__main__":

@@ -681,7 +681,7 @@ struct FrontRunner {
             hipLaunchKernelGGL(k_big_diag<T>, dim3(nb), dim3(256), 0, s, a, beg, k0, front);
             int const rest = mmax - std::min(k0 + kPanel, pmax);
             if (rest <= 0) continue;
-            hipLaunchKernelGGL(k_big_trsm_rows<T>, dim3(blocks_for(rest, 256), nb), dim3(256), 0, s, a, beg, k0, front);
+            hipLaunchKernelGGL(k_big_trsm_rows<T>, dim3(blocks_for(rest, 64), nb), dim3(256), 0, s, a, beg, k0, front);
             hipLaunchKernelGGL(k_big_trsm<T>, dim3(blocks_for(rest, 256), nb), dim3(256), 0, s, a, beg, k0, kPanel,
                                front);
             int const tiles = blocks_for(rest, 128);

@@ -492,52 +492,51 @@ __global__ void __launch_bounds__(256) k_big_diag(FrontArgs a, int begin, int k0
     __syncthreads();
     for (int k = 0; k < kb; ++k) {
         T const rp = pivot_recip(D[k][k], a.fail, fm.inst, tid == 0);
-        for (int i = k + 1 + tid; i < kb; i += 256) {
-            T const l = D[i][k] * rp;
-            D[i][k] = l;
-            for (int j = k + 1; j < kb; ++j) D[i][j] -= l * D[k][j];
+        int const c = kb - k - 1;
+        if (tid < c) D[k + 1 + tid][k] *= rp;  // the multipliers l_ik
+        __syncthreads();
+        for (int e = tid; e < c * c; e += 256) {
+            int const i = k + 1 + e / c, j = k + 1 + e % c;
+            D[i][j] -= D[i][k] * D[k][j];
         }
         __syncthreads();
     }
     for (int e = tid; e < kb * kb; e += 256) F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb] = D[e / kb][e % kb];
 }
 
-// L21 = A21 U11^-1 for the rows below the panel: a thread per row, the row's
-// panel segment in registers, U11 in LDS.
+// L21 = A21 U11^-1 for the rows below the panel: 64 rows per workgroup staged
+// in LDS with coalesced loads, a thread per row for the substitution, U11 in LDS.
 template <typename T>
 __global__ void __launch_bounds__(256) k_big_trsm_rows(FrontArgs a, int begin, int k0, T* front) {
+    constexpr int RB = 64;
     __shared__ T U[kBigPanel][kBigPanel + 1];
-    __shared__ T R[kBigPanel];
+    __shared__ T R[RB][kBigPanel + 1];
+    __shared__ T Rp[kBigPanel];
     FrontMeta const fm = a.meta[begin + blockIdx.y];
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
     if (k0 >= p) return;
     int const kb = min(kBigPanel, p - k0), kend = k0 + kb;
+    int const r0 = kend + static_cast<int>(blockIdx.x) * RB;
+    if (r0 >= m) return;  // uniform: no rows for this block
+    int const nr = min(RB, m - r0);
     int const tid = threadIdx.x;
     T* F = front + fm.off;
-    if (kend + static_cast<int>(blockIdx.x) * 256 >= m) return;  // uniform: no rows for this block
     for (int e = tid; e < kb * kb; e += 256) U[e / kb][e % kb] = F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb];
+    for (int e = tid; e < nr * kb; e += 256) R[e / kb][e % kb] = F[static_cast<long long>(r0 + e / kb) * m + k0 + e % kb];
     __syncthreads();
-    if (tid < kb) R[tid] = pivot_recip(U[tid][tid], nullptr, 0, false);
+    if (tid < kb) Rp[tid] = pivot_recip(U[tid][tid], nullptr, 0, false);
     __syncthreads();
-    int const i = kend + blockIdx.x * 256 + tid;
-    if (i >= m) return;
-    T* ri = F + static_cast<long long>(i) * m + k0;
-    T x[kBigPanel];
-#pragma unroll
-    for (int k = 0; k < kBigPanel; ++k) x[k] = k < kb ? ri[k] : T(0);
-#pragma unroll
-    for (int k = 0; k < kBigPanel; ++k) {
-        if (k < kb) {
+    if (tid < nr) {
+        T* x = R[tid];
+        for (int k = 0; k < kb; ++k) {
             T acc = x[k];
-#pragma unroll
             for (int q = 0; q < k; ++q) acc -= x[q] * U[q][k];
-            x[k] = acc * R[k];
+            x[k] = acc * Rp[k];
         }
     }
-#pragma unroll
-    for (int k = 0; k < kBigPanel; ++k)
-        if (k < kb) ri[k] = x[k];
+    __syncthreads();
+    for (int e = tid; e < nr * kb; e += 256) F[static_cast<long long>(r0 + e / kb) * m + k0 + e % kb] = R[e / kb][e % kb];
 }
 
 // U12 = L11^-1 A12 for the columns right of the panel: a thread per column,
