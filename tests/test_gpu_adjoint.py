@@ -186,6 +186,53 @@ def test_adjoint_vs_finite_differences(device, oracle, order, h_rel, bar):
     assert errs.max() < bar
 
 
+def test_adjoint_10k_grid_scenarios(device, oracle):
+    """10k-bus meshed grid (1,500 transformers, 300 shunts, 15% PV) x 4 load
+    scenarios (+-30%) in one plan: the FP64 adjoint against the sparse FP64
+    reference gradient (scenario 0: every load, 300 random branches) and
+    against eighth-order central differences of full re-solves on 16
+    parameters of scenario 1."""
+    from adjoint_ref import reference_gradient
+    base = 100e6
+    net = gridgen.large_mesh(10_000, seed=1)
+    scen = gridgen.load_scenarios(net, 4, seed=2000)
+    plan = capi.Plan(scen)
+    r = plan.solve(1e-11, 60, base)
+    assert r["converged"].all()
+    out = plan.adjoint(1.0, 1.0)
+    nb, nr = net.n_bus, net.n_branch
+    rng = np.random.default_rng(11)
+    brs = rng.choice(nr, 300, replace=False)
+    gp, gq, gr, gx, L = reference_gradient(oracle, scen[0], r["v"][0], base, 1.0, 1.0, branches=brs)
+    errs = (rel_err(out["grad_p"][:nb], gp), rel_err(out["grad_q"][:nb], gq),
+            rel_err(out["grad_r"][:nr][brs], gr[brs]), rel_err(out["grad_x"][:nr][brs], gx[brs]))
+    print("10k adjoint vs sparse reference, rel err p q r x:", errs)
+    assert abs(out["loss"][0] - L) <= 1e-11 * abs(L)
+    assert max(errs) < 1e-6
+    # eighth-order central differences of re-solves on scenario 1, at the
+    # parameters with the largest gradients (where a difference quotient can
+    # resolve 1e-6) and at random ones (held to the measured noise floor)
+    from fd_check import GRAD_KEY, assess, fd_gradients, noise_floor
+    g1 = {k: out[k][nb:2 * nb] if k in ("grad_p", "grad_q") else out[k][nr:2 * nr] for k in
+          ("grad_p", "grad_q", "grad_r", "grad_x")}
+    pq_buses = np.nonzero(scen[1].bus_type == PQ)[0]
+    params = [(1, "br_r", int(k)) for k in np.argsort(-np.abs(g1["grad_r"]))[:3]]
+    params += [(1, "br_x", int(k)) for k in np.argsort(-np.abs(g1["grad_x"]))[:3]]
+    params += [(1, "p_w", int(i)) for i in pq_buses[np.argsort(-np.abs(g1["grad_p"][pq_buses]))[:2]]]
+    params += [(1, "p_w", int(i)) for i in rng.choice(pq_buses, 2, replace=False)]
+    params += [(1, "q_var", int(i)) for i in rng.choice(pq_buses, 2, replace=False)]
+    params += [(1, arr, int(k)) for arr in ("br_r", "br_x") for k in rng.choice(nr, 2, replace=False)]
+    g_fd, hs = fd_gradients(oracle, scen, params, 8, 3e-2, base)
+    sigma = noise_floor(oracle, scen[1], base)
+    g_adj = np.array([g1[GRAD_KEY[a]][k] for _, a, k in params])
+    rep = assess(g_adj, g_fd, hs, sigma, 8)
+    print("10k adjoint vs 8th-order FD (h = 3e-2 rel): noise floor of L", sigma,
+          {k: v for k, v in rep.items() if k not in ("ok", "resolvable_mask")})
+    assert rep["ok"].all()
+    assert rep["resolvable_mask"][:6].sum() >= 2
+    assert rep["resolvable_max_rel"] < 1e-6
+
+
 def test_adjoint_batch_equals_single(device):
     base = 100e6
     net = gridgen.small_mesh(500, seed=12, n_extra=40)
