@@ -226,6 +226,7 @@ struct DevFronts {
     int fz_pool = 0, fz_ld4 = 1;
     DBuf<int> fz_order, fz_lvl_beg, fz_lvl_ptr;
     DBuf<GridTask> fz_tasks;
+    DBuf<int4> fz_gfront;  // per front: storage offset, vector offset, m | p << 16, first slot
     DBuf<int4> fz_pairs;
     DBuf<double2> fz_ypair;  // Y value of each pair, refreshed at every solve
     long long fz_npairs = 0;
@@ -435,21 +436,27 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     });
     d.fz_work.resize(G);
     for (int k = 0; k < G; ++k) d.fz_work[k] = d.inst_factor[order[k]] + d.inst_slots[order[k]];
-    auto nonempty = [](auto& v, auto z) {
-        if (v.empty()) v.push_back(z);
-    };
-    nonempty(tasks, GridTask{});
-    nonempty(pairs, make_int4(0, 0, 0, 0));
-    nonempty(rhs, make_int2(0, 0));
-    nonempty(ext, make_int2(0, 0));
-    nonempty(vx, make_int2(0, 0));
-    nonempty(sol, make_int2(0, 0));
+    // compact per-front records of the grid kernel (offsets fit in int: fz_ok)
+    std::vector<int4> gfront(nf);
+    for (int n = 0; n < nf; ++n) {
+        FrontMeta const& fm = meta[n];
+        if (fm.off < 0 || fm.off > kIntMax || fm.voff < 0 || fm.voff > kIntMax || fm.m < 0 || fm.m > 0x7fff ||
+            fm.p < 0 || fm.p > fm.m)
+            throw Error(GAPHIP_E_ARG, "front record out of range");
+        gfront[n] = make_int4(static_cast<int>(fm.off), static_cast<int>(fm.voff), fm.m | (fm.p << 16), fm.slot_beg);
+    }
+    // padding read by the grid kernel's unconditional loads (never executed)
+    tasks.insert(tasks.end(), kTaskPad, GridTask{});
+    gfront.insert(gfront.end(), kWave, make_int4(0, 0, 0, 0));
+    pairs.insert(pairs.end(), kListPad, make_int4(0, 0, 0, 0));
+    for (auto* v : {&rhs, &ext, &vx, &sol}) v->insert(v->end(), kListPad, make_int2(0, -1));
     d.fz_list_bytes = static_cast<long long>(tasks.size()) * sizeof(GridTask) + pairs.size() * sizeof(int4) +
                       (rhs.size() + ext.size() + vx.size() + sol.size()) * sizeof(int2);
     d.fz_order.upload(order, s);
     d.fz_lvl_beg.upload(lvl_beg, s);
     d.fz_lvl_ptr.upload(lvl_ptr, s);
     d.fz_tasks.upload(tasks, s);
+    d.fz_gfront.upload(gfront, s);
     d.fz_pairs.upload(pairs, s);
     d.fz_npairs = static_cast<long long>(pairs.size());
     d.fz_ypair.alloc(d.fz_npairs);
@@ -660,6 +667,7 @@ void pack_fronts(std::vector<LUPlan const*> const& plans, std::vector<long long>
     d.arrow_src.upload(arrow_src, s);
     d.arrow_dst.upload(arrow_dst, s);
     d.slot_row.upload(slot_row, s);
+    slot_col.insert(slot_col.end(), kVecPad, 0);  // read past a front's last slot by the grid kernel
     d.slot_col.upload(slot_col, s);
     d.bigbase.upload(bigbase, s);
     d.arow.upload(arow, s);
@@ -1085,6 +1093,7 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     P.app_g.upload(app_g, s);
     P.app_b.upload(app_b, s);
     P.y_rp.upload(y_rp, s);
+    y_ci.insert(y_ci.end(), kRowPad, 0);  // the grid mismatch reads rows in steps of kRowPad entries
     P.y_ci.upload(y_ci, s);
     P.y_seg.upload(y_seg, s);
     P.y_code.upload(y_code, s);
@@ -1148,7 +1157,8 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     P.vpol.alloc(P.n_bus);
     P.s.alloc(P.n_bus);
     P.sspec.alloc(P.n_bus);
-    P.y_pu.alloc(P.y_nnz);
+    P.y_pu.alloc(P.y_nnz + kRowPad);
+    HIPCHK(hipMemsetAsync(P.y_pu.p + P.y_nnz, 0, sizeof(double2) * kRowPad, s));
     P.zb.alloc(n_inst);
     P.done.alloc(n_inst);
     P.iters.alloc(n_inst);
@@ -1158,12 +1168,12 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     P.dx.alloc(std::max<long long>(P.n_var, 1));
     if (precision == GAPHIP_FP64) {
         P.jv64.alloc(std::max<long long>(P.j_nnz, 1));
-        P.front64.alloc(std::max<long long>(P.fr.front_entries, 1));
-        P.vec64.alloc(std::max<long long>(P.fr.vec_entries, 1));
+        P.front64.alloc(P.fr.front_entries + kFrontPad);
+        P.vec64.alloc(P.fr.vec_entries + kVecPad);
     } else {
         P.jv32.alloc(std::max<long long>(P.j_nnz, 1));
-        P.front32.alloc(std::max<long long>(P.fr.front_entries, 1));
-        P.vec32.alloc(std::max<long long>(P.fr.vec_entries, 1));
+        P.front32.alloc(P.fr.front_entries + kFrontPad);
+        P.vec32.alloc(P.fr.vec_entries + kVecPad);
     }
     P.analysis_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -1377,7 +1387,7 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
         hipLaunchKernelGGL(k_gather_ypair, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
                            static_cast<int>(d.fz_npairs), d.fz_pairs.p, P.y_pu.p, d.fz_ypair.p);
         FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
-                     d.fz_tasks.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
+                     d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
                      P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, P.dx.p};
         MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
                    P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p};
@@ -1437,8 +1447,8 @@ void run_adjoint(gaphip_plan& P, double w_loss, double w_dev, double const* g_vm
     int const G = P.n_inst;
     long long const N = P.n_bus;
     if (!P.jv64.p) P.jv64.alloc(std::max<long long>(P.j_nnz, 1));
-    if (!P.front64.p) P.front64.alloc(std::max<long long>(P.fr.front_entries, 1));
-    if (!P.vec64.p) P.vec64.alloc(std::max<long long>(P.fr.vec_entries, 1));
+    if (!P.front64.p) P.front64.alloc(P.fr.front_entries + kFrontPad);
+    if (!P.vec64.p) P.vec64.alloc(P.fr.vec_entries + kVecPad);
     P.adj_lossb.alloc(N);
     P.adj_gp.alloc(N);
     P.adj_gq.alloc(N);

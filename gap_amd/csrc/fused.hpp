@@ -46,6 +46,7 @@ struct FusedArgs {
     const int* lvl_beg;   // per network: its levels are [lvl_beg[g], lvl_beg[g+1]) of lvl_ptr
     const int* lvl_ptr;   // per (network, level): tasks [lvl_ptr[e], lvl_ptr[e+1])
     const GridTask* tasks;
+    const int4* gfront;   // per front: (storage offset, vector offset, m | p << 16, first slot)
     const int4* pairs;    // (i, j, Y slot, LDS position | entry mask << 16)
     const double2* ypair; // per-unit Y value of each pair, in pair order (gathered once per solve)
     const int2* rhs;      // (equation index, LDS position)
@@ -95,10 +96,37 @@ __device__ inline double2 ld_stream(const double2* p) {
     return make_double2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
 }
 
+// Lane index, recomputed at each call: an empty volatile asm keeps the
+// compiler from hoisting the lane-dependent masks and addresses of every task
+// class out of the Newton loop, where they would stay live for the whole kernel.
+__device__ inline int lane_id() {
+    int l = static_cast<int>(threadIdx.x);
+    asm volatile("" : "+v"(l));
+    return l & (kWave - 1);
+}
+
 template <typename T>
 __device__ inline void lds_add(T* p, T v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+
+// Chunks of 64 list entries each lane keeps in registers, so that all index
+// loads of a task are issued together, then all value loads; longer lists
+// finish in a loop.
+constexpr int kPairChunks = 1, kExtChunks = 2;
+
+// Every load of the grid kernel is issued for every lane, with no bounds
+// branch around it: a load under a divergent branch is waited for where the
+// branch joins, which serialises a task's gathers.  The buffers it reads
+// therefore extend past their last element (padding entries are neutral:
+// index 0, LDS position -1, mask 0), and the bounds are applied where the
+// values are used.
+constexpr int kListPad = 2 * kWave;             // entries after each task list
+constexpr int kTaskPad = 16;                    // descriptors after the last task (2 ahead at 8 wavefronts)
+constexpr int kRowPad = 4;                      // Y-bus entries after the last row
+constexpr long long kFrontPad = kWave * kWave;  // front storage: one 64 x 64 front
+constexpr long long kVecPad = kWave;            // slot vectors and slot columns
+static_assert(kListPad >= kPairChunks * kWave && kListPad >= kExtChunks * kWave, "list padding too short");
 
 // ---------------------------------------------------------------------------
 // Rectangular forms of the per-bus and per-pair arithmetic.  They equal the
@@ -114,18 +142,21 @@ __device__ inline double grid_mismatch_bus(MisArgs const& a, int i) {
     int const q1 = a.rp[i + 1];
     // four row entries per step: their loads are issued together, the sums
     // keep the row order
-    for (int q = a.rp[i]; q < q1; q += 4) {
-        int c[4];
-        double2 yv[4], vv[4];
+    for (int q = a.rp[i]; q < q1; q += kRowPad) {
+        // unconditional loads (the Y-bus arrays extend kRowPad entries past
+        // the last row, and every column index is a bus); entries past the
+        // row are not summed
+        int c[kRowPad];
+        double2 yv[kRowPad], vv[kRowPad];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            c[u] = q + u < q1 ? a.ci[q + u] : i;
-            yv[u] = q + u < q1 ? a.y[q + u] : make_double2(0.0, 0.0);
+        for (int u = 0; u < kRowPad; ++u) {
+            c[u] = a.ci[q + u];
+            yv[u] = a.y[q + u];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) vv[u] = a.v[c[u]];
+        for (int u = 0; u < kRowPad; ++u) vv[u] = a.v[c[u]];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < kRowPad; ++u)
             if (q + u < q1) {
                 cr += yv[u].x * vv[u].x - yv[u].y * vv[u].y;
                 cim += yv[u].x * vv[u].y + yv[u].y * vv[u].x;
@@ -198,8 +229,9 @@ __device__ inline void sincos_small(double x, double& sn, double& cs) {
 __device__ inline void grid_update_bus(UpdArgs const& a, int i, bool zero) {
     double2 v = a.v[i];
     int av = a.ang[i], mv = a.mag[i];
+    double const dxa = a.dx[av < 0 ? 0 : av], dxm = a.dx[mv < 0 ? 0 : mv];  // unconditional loads
     if (av >= 0) {
-        double ac = 0.9 * (zero ? 0.0 : a.dx[av]);
+        double ac = 0.9 * (zero ? 0.0 : dxa);
         if (fabs(ac) > 0.15) ac = 0.15 * (ac > 0 ? 1.0 : -1.0);
         double sn, cs;
         sincos_small(ac, sn, cs);
@@ -207,7 +239,7 @@ __device__ inline void grid_update_bus(UpdArgs const& a, int i, bool zero) {
     }
     if (mv >= 0) {
         double const old = sqrt(v.x * v.x + v.y * v.y);
-        double mc = 0.9 * (zero ? 0.0 : a.dx[mv]);
+        double mc = 0.9 * (zero ? 0.0 : dxm);
         if (fabs(mc) > 0.08) mc = 0.08 * (mc > 0 ? 1.0 : -1.0);
         double vm = old + mc;
         vm = fmax(vm, 0.5);
@@ -217,61 +249,85 @@ __device__ inline void grid_update_bus(UpdArgs const& a, int i, bool zero) {
     a.v[i] = v;
 }
 
-// Chunks of 64 list entries each lane keeps in registers, so that all index
-// loads of a task are issued together, then all value loads; longer lists
-// finish in a loop.
-constexpr int kPairChunks = 1, kExtChunks = 2;
+
+// The loads of a task that depend only on its descriptor (the lane's front
+// record and list entries, the pairs' Y values).  A wavefront issues them one
+// task ahead, so a task waits only for its gathers of V, S, F and child data.
+struct TaskPre {
+    int4 gf;                 // front record of the lane's front
+    int4 pr[kPairChunks];    // pair list entries
+    double2 yv[kPairChunks]; // their per-unit Y
+    int2 ex[kExtChunks];     // extend-add entries
+    int2 rh, vv;             // right-hand-side and child-vector entries
+    int2 so;                 // (backward) solution entries
+};
+
+// Unconditional: entries past the task's lists are the next task's or padding.
+template <bool BWD>
+__device__ inline TaskPre load_pre(FusedArgs const& a, GridTask const& tk) {
+    int const lane = lane_id(), cls = tk.cls_cnt >> 8;
+    int const gi = lane >> (2 + cls);  // 4 << cls lanes per front
+    TaskPre r;
+    r.gf = ld_stream(a.gfront + tk.front_beg + gi);
+    if constexpr (BWD) {
+        r.so = ld_stream(a.sol + tk.sol_beg + lane);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kPairChunks; ++k) {
+            r.pr[k] = ld_stream(a.pairs + tk.pair_beg + k * kWave + lane);
+            r.yv[k] = ld_stream(a.ypair + tk.pair_beg + k * kWave + lane);
+        }
+#pragma unroll
+        for (int k = 0; k < kExtChunks; ++k) r.ex[k] = ld_stream(a.ext + tk.ext_beg + k * kWave + lane);
+        r.rh = ld_stream(a.rhs + tk.rhs_beg + lane);
+        r.vv = ld_stream(a.vx + tk.vx_beg + lane);
+    }
+    return r;
+}
 
 // Assemble, factor and forward-eliminate up to kWave / L fronts of one class.
 // Stores the U rows (diagonal onwards), the Schur complement and the
 // eliminated right-hand side w of every front.
 template <typename T, int MM, int L>
-__device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, GridTask const& tk,
-                                 T* front, T* vec, T* pool, int ld) {
+__device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, GridTask const& tk,
+                                 TaskPre const& pre, int g, T* front, T* vec, T* pool, int ld) {
     constexpr int G = kWave / L;
-    int const lane = threadIdx.x % kWave, gi = lane / L, t = lane % L;
+    int const lane = lane_id(), gi = lane / L, t = lane % L;
     int const rows = ld - 1, wbase = G * rows * ld, region = wbase + G * MM;
-    // indices
-    FrontMeta fm{};
-    if (gi < (tk.cls_cnt & 0xff)) fm = fa.meta[tk.front_beg + gi];
-    int4 pr[kPairChunks];
+    int4 const* pr = pre.pr;
+    int2 const* ex = pre.ex;
+    int2 const rh = pre.rh, vv = pre.vv;
+    double2 const* yv = pre.yv;
+    // gathers, unconditional: every list entry (the next task's and padding
+    // included) holds valid indices; which entries are this task's is applied
+    // where the values are used
+    double2 vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) {
-        int const q = tk.pair_beg + k * kWave + lane;
-        pr[k] = q < tk.pair_end ? ld_stream(a.pairs + q) : make_int4(0, 0, 0, 0);
-    }
-    int2 ex[kExtChunks];
-#pragma unroll
-    for (int k = 0; k < kExtChunks; ++k) {
-        int const q = tk.ext_beg + k * kWave + lane;
-        ex[k] = q < tk.ext_end ? ld_stream(a.ext + q) : make_int2(0, -1);
-    }
-    int2 const rh = tk.rhs_beg + lane < tk.rhs_end ? ld_stream(a.rhs + tk.rhs_beg + lane) : make_int2(0, -1);
-    int2 const vv = tk.vx_beg + lane < tk.vx_end ? ld_stream(a.vx + tk.vx_beg + lane) : make_int2(0, -1);
-    // values
-    double2 yv[kPairChunks], vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
-#pragma unroll
-    for (int k = 0; k < kPairChunks; ++k) {
-        bool const live = (pr[k].w >> 16) != 0, diag = pr[k].x == pr[k].y;
-        double2 const z = make_double2(0.0, 0.0);
-        yv[k] = live ? ld_stream(a.ypair + tk.pair_beg + k * kWave + lane) : z;
-        vi[k] = live ? ma.v[pr[k].x] : z;
-        vj[k] = live && !diag ? ma.v[pr[k].y] : z;
-        si[k] = live && diag ? ma.s[pr[k].x] : z;
+        vi[k] = ma.v[pr[k].x];
+        vj[k] = ma.v[pr[k].y];
+        si[k] = ma.s[pr[k].x];
     }
     T exv[kExtChunks];
 #pragma unroll
-    for (int k = 0; k < kExtChunks; ++k) exv[k] = ex[k].y >= 0 ? ld_stream(front + ex[k].x) : T(0);
-    double const fv = rh.y >= 0 ? ma.f[rh.x] : 0.0;
-    T const vxv = vv.y >= 0 ? vec[vv.x] : T(0);
+    for (int k = 0; k < kExtChunks; ++k) exv[k] = ld_stream(front + ex[k].x);
+    double const fv = ma.f[rh.x];
+    T const vxv = vec[vv.x];
+    bool pok[kPairChunks], eok[kExtChunks];
+#pragma unroll
+    for (int k = 0; k < kPairChunks; ++k) pok[k] = tk.pair_beg + k * kWave + lane < tk.pair_end;
+#pragma unroll
+    for (int k = 0; k < kExtChunks; ++k) eok[k] = tk.ext_beg + k * kWave + lane < tk.ext_end && ex[k].y >= 0;
+    bool const rok = tk.rhs_beg + lane < tk.rhs_end && rh.y >= 0;
+    bool const vok = tk.vx_beg + lane < tk.vx_end && vv.y >= 0;
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
     wave_sync();  // the previous task of this wavefront has finished with the region
     for (int e = lane; e < region; e += kWave) pool[e] = T(0);
     wave_sync();
-    auto put_pair = [&](int4 const& p4, double2 y2, double2 pi2, double2 pj2, double2 s2) {
+    auto put_pair = [&](bool ok, int4 const& p4, double2 y2, double2 pi2, double2 pj2, double2 s2) {
         int const base = p4.w & 0xffff, mask = p4.w >> 16;
-        if (!mask) return;
+        if (!ok || !mask) return;
         double pt, pv, qt, qv;
         jac_block_rect(p4.x == p4.y, y2, pi2, pj2, s2, pt, pv, qt, qv);
         if (mask & 1) pool[base] = static_cast<T>(pt);
@@ -280,14 +336,12 @@ __device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArg
         if (mask & 8) pool[base + ld + 1] = static_cast<T>(qv);
     };
 #pragma unroll
-    for (int k = 0; k < kPairChunks; ++k) put_pair(pr[k], yv[k], vi[k], vj[k], si[k]);
+    for (int k = 0; k < kPairChunks; ++k) put_pair(pok[k], pr[k], yv[k], vi[k], vj[k], si[k]);
     for (int q = tk.pair_beg + kPairChunks * kWave + lane; q < tk.pair_end; q += kWave) {
         int4 const p4 = ld_stream(a.pairs + q);
-        bool const diag = p4.x == p4.y;
-        double2 const z = make_double2(0.0, 0.0);
-        put_pair(p4, ld_stream(a.ypair + q), ma.v[p4.x], diag ? z : ma.v[p4.y], diag ? ma.s[p4.x] : z);
+        put_pair(true, p4, ld_stream(a.ypair + q), ma.v[p4.x], ma.v[p4.y], ma.s[p4.x]);
     }
-    if (rh.y >= 0) pool[rh.y] = static_cast<T>(-1.0 * fv);
+    if (rok) pool[rh.y] = static_cast<T>(-1.0 * fv);
     for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave) {
         int2 const e = ld_stream(a.rhs + q);
         pool[e.y] = static_cast<T>(-1.0 * ma.f[e.x]);
@@ -295,18 +349,19 @@ __device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArg
     wave_sync();
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k)
-        if (ex[k].y >= 0) lds_add(pool + ex[k].y, exv[k]);
+        if (eok[k]) lds_add(pool + ex[k].y, exv[k]);
     for (int q = tk.ext_beg + kExtChunks * kWave + lane; q < tk.ext_end; q += kWave) {
         int2 const e = ld_stream(a.ext + q);
         lds_add(pool + e.y, ld_stream(front + e.x));
     }
-    if (vv.y >= 0) lds_add(pool + vv.y, vxv);
+    if (vok) lds_add(pool + vv.y, vxv);
     for (int q = tk.vx_beg + kWave + lane; q < tk.vx_end; q += kWave) {
         int2 const e = ld_stream(a.vx + q);
         lds_add(pool + e.y, vec[e.x]);
     }
     wave_sync();
-    int const m = fm.m, p = fm.p;
+    bool const act = gi < (tk.cls_cnt & 0xff);
+    int const m = act ? pre.gf.z & 0xffff : 0, p = act ? pre.gf.z >> 16 : 0;
     int const p_max = wave_group_max<L>(p);
     if constexpr (MM <= 16) {
         // dense elimination in registers: lane t holds column t (rows 0..MM-1)
@@ -326,7 +381,7 @@ __device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArg
             bool const act = k < p;
             T piv = act ? lk[k] : T(1);
             if (tabs(piv) < T(1e-14)) {
-                if (act && t == 0) fa.fail[fm.inst] = 1;
+                if (act && t == 0) fa.fail[g] = 1;
                 piv = T(1);
             }
             T const rp = T(1) / piv;
@@ -342,11 +397,11 @@ __device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArg
             }
         }
         if (t < m) {
-            T* Gm = front + fm.off;
+            T* Gm = front + pre.gf.x;
 #pragma unroll
             for (int r = 0; r < MM; ++r)
                 if (r < m && t >= (r < p ? r : p)) Gm[r * m + t] = col[r];
-            vec[fm.voff + t] = wt;
+            vec[pre.gf.y + t] = wt;
         }
         return;
     }
@@ -354,7 +409,7 @@ __device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArg
         T piv = k < p ? F[k * ld + k] : T(1);
         bool const tiny = tabs(piv) < T(1e-14);
         if (tiny) {
-            if (k < p && t == 0) fa.fail[fm.inst] = 1;
+            if (k < p && t == 0) fa.fail[g] = 1;
             piv = T(1);
         }
         T const rp = T(1) / piv;
@@ -369,43 +424,50 @@ __device__ void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArg
     }
     wave_sync();
     if (t < m) {
-        T* Gm = front + fm.off;
+        T* Gm = front + pre.gf.x;
         for (int r = 0; r < m; ++r)
             if (t >= (r < p ? r : p)) Gm[r * m + t] = F[r * ld + t];
-        vec[fm.voff + t] = w[t];
+        vec[pre.gf.y + t] = w[t];
     }
 }
 
 // Backward substitution of the pivot block of up to kWave / L fronts.
 template <typename T, int MM, int L>
-__device__ void grid_backward_task(FusedArgs const& a, FrontArgs const& fa, GridTask const& tk, T const* front,
-                                   T const* vec, double* sol, T* pool, int ld) {
+__device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs const& a, FrontArgs const& fa, GridTask const& tk, TaskPre const& pre,
+                                   T const* front, T const* vec, double* sol, T* pool, int ld) {
     constexpr int G = kWave / L, CH = MM < 16 ? MM : 16;
-    int const lane = threadIdx.x % kWave, gi = lane / L, t = lane % L;
+    int const lane = lane_id(), gi = lane / L, t = lane % L;
     int const rows = ld - 1, wbase = G * rows * ld;
-    FrontMeta fm{};
-    if (gi < (tk.cls_cnt & 0xff)) fm = fa.meta[tk.front_beg + gi];
-    int2 const so = tk.sol_beg + lane < tk.sol_end ? ld_stream(a.sol + tk.sol_beg + lane) : make_int2(0, -1);
-    T const sv = so.y >= 0 ? static_cast<T>(sol[so.x]) : T(0);
-    int const m = fm.m, p = fm.p;
-    T const wv = t < p ? vec[fm.voff + t] : T(0);
+    int2 const so = pre.so;
+    bool const sok = tk.sol_beg + lane < tk.sol_end && so.y >= 0;
+    bool const act = gi < (tk.cls_cnt & 0xff);
+    int const m = act ? pre.gf.z & 0xffff : 0, p = act ? pre.gf.z >> 16 : 0;
+    // unconditional loads: front records of inactive lanes are another
+    // front's or padding, and vectors, slot columns and front storage extend
+    // one 64-slot front past their end
+    T const sv = static_cast<T>(sol[so.x]);
+    T const wv = vec[pre.gf.y + t];
+    int const colt = fa.slot_col[pre.gf.w + t];
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
-    const int* cols = fa.slot_col + fm.slot_beg;
     wave_sync();  // the previous task of this wavefront has finished with the region
-    if (so.y >= 0) pool[so.y] = sv;
+    if (sok) pool[so.y] = sv;
     for (int q = tk.sol_beg + kWave + lane; q < tk.sol_end; q += kWave) {
         int2 const e = ld_stream(a.sol + q);
         pool[e.y] = static_cast<T>(sol[e.x]);
     }
     if (t < p) w[t] = wv;
+    T xme = T(0);  // this lane's solution value (t < p)
     if constexpr (MM <= 16) {
         // lane t < p holds row t of U; x of the structure slots and the steps'
         // x_k come from the owning lanes by shuffles
         T urow[MM];
-        const T* Gr = front + fm.off + static_cast<long long>(t) * m;
+        const T* Gr = front + pre.gf.x + static_cast<long long>(t) * (pre.gf.z & 0xffff);
 #pragma unroll
-        for (int q = 0; q < MM; ++q) urow[q] = (t < p && q < m) ? ld_stream(Gr + q) : T(0);
+        for (int q = 0; q < MM; ++q) {
+            T const x = ld_stream(Gr + q);
+            urow[q] = (t < p && q < m) ? x : T(0);
+        }
         wave_sync();
         T acc = t < m ? w[t] : T(0);
         T xs[MM];
@@ -423,13 +485,14 @@ __device__ void grid_backward_task(FusedArgs const& a, FrontArgs const& fa, Grid
                 T const xk = __shfl(acc / urow[k], gi * L + k);
                 if (k < p) {
                     if (t < k) acc -= urow[k] * xk;
-                    if (t == k) sol[cols[k]] = static_cast<double>(xk);
+                    if (t == k) xme = xk;
                 }
             }
         }
+        if (t < p) sol[colt] = static_cast<double>(xme);
         return;
     }
-    load_rows<T, MM, L, CH>(F, ld, front + fm.off, m, p, t, t < m);
+    load_rows<T, MM, L, CH>(F, ld, front + pre.gf.x, m, p, t, t < m);
     wave_sync();
     if (t < p) {
         T acc = w[t];
@@ -442,35 +505,36 @@ __device__ void grid_backward_task(FusedArgs const& a, FrontArgs const& fa, Grid
         if (k < p) {
             T const xk = w[k] / F[k * ld + k];
             if (t < k) w[t] -= F[t * ld + k] * xk;
-            if (t == k) sol[cols[k]] = static_cast<double>(xk);
+            if (t == k) xme = xk;
         }
         wave_sync();
     }
+    if (t < p) sol[colt] = static_cast<double>(xme);
 }
 
 template <typename T, bool BWD>
-__device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, GridTask const& tk,
-                                 T* front, T* vec, T* pool) {
+__device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int g, GridTask const& tk,
+                                 TaskPre const& pre, T* front, T* vec, T* pool) {
     switch (tk.cls_cnt >> 8) {
         case 0:
-            if constexpr (BWD) grid_backward_task<T, 4, 4>(a, fa, tk, front, vec, a.dx, pool, 5);
-            else grid_factor_task<T, 4, 4>(a, fa, ma, tk, front, vec, pool, 5);
+            if constexpr (BWD) grid_backward_task<T, 4, 4>(a, fa, tk, pre, front, vec, a.dx, pool, 5);
+            else grid_factor_task<T, 4, 4>(a, fa, ma, tk, pre, g, front, vec, pool, 5);
             break;
         case 1:
-            if constexpr (BWD) grid_backward_task<T, 8, 8>(a, fa, tk, front, vec, a.dx, pool, 9);
-            else grid_factor_task<T, 8, 8>(a, fa, ma, tk, front, vec, pool, 9);
+            if constexpr (BWD) grid_backward_task<T, 8, 8>(a, fa, tk, pre, front, vec, a.dx, pool, 9);
+            else grid_factor_task<T, 8, 8>(a, fa, ma, tk, pre, g, front, vec, pool, 9);
             break;
         case 2:
-            if constexpr (BWD) grid_backward_task<T, 16, 16>(a, fa, tk, front, vec, a.dx, pool, 17);
-            else grid_factor_task<T, 16, 16>(a, fa, ma, tk, front, vec, pool, 17);
+            if constexpr (BWD) grid_backward_task<T, 16, 16>(a, fa, tk, pre, front, vec, a.dx, pool, 17);
+            else grid_factor_task<T, 16, 16>(a, fa, ma, tk, pre, g, front, vec, pool, 17);
             break;
         case 3:
-            if constexpr (BWD) grid_backward_task<T, 32, 32>(a, fa, tk, front, vec, a.dx, pool, 33);
-            else grid_factor_task<T, 32, 32>(a, fa, ma, tk, front, vec, pool, 33);
+            if constexpr (BWD) grid_backward_task<T, 32, 32>(a, fa, tk, pre, front, vec, a.dx, pool, 33);
+            else grid_factor_task<T, 32, 32>(a, fa, ma, tk, pre, g, front, vec, pool, 33);
             break;
         default:
-            if constexpr (BWD) grid_backward_task<T, 64, 64>(a, fa, tk, front, vec, a.dx, pool, a.ld4);
-            else grid_factor_task<T, 64, 64>(a, fa, ma, tk, front, vec, pool, a.ld4);
+            if constexpr (BWD) grid_backward_task<T, 64, 64>(a, fa, tk, pre, front, vec, a.dx, pool, a.ld4);
+            else grid_factor_task<T, 64, 64>(a, fa, ma, tk, pre, g, front, vec, pool, a.ld4);
     }
 }
 
@@ -480,17 +544,31 @@ __global__ void __launch_bounds__(256) k_gather_ypair(int n, const int4* pairs, 
     if (k < n) ypair[k] = y[pairs[k].z];
 }
 
-// This wavefront's tasks q, q + NW, ... < qe of one level; the next
-// descriptor is loaded while the current task runs.
+// This wavefront's tasks q, q + NW, ... < qe of one level (q is uniform over
+// the wavefront).  While a task runs, the next task's descriptor-dependent
+// loads and the descriptor after that are in flight (unrolled by two, so no
+// value is copied while its load is outstanding).  The descriptors read past
+// qe are the next level's or padding; only their static lists are loaded.
 template <typename T, bool BWD, int NW>
-__device__ inline void run_level(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int q, int qe, T* front,
-                                 T* vec, T* pool) {
+__device__ inline void run_level(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int g, int q, int qe,
+                                 T* front, T* vec, T* pool) {
     if (q >= qe) return;
-    GridTask nxt = a.tasks[q];
-    for (; q < qe; q += NW) {
-        GridTask const cur = nxt;
-        if (q + NW < qe) nxt = a.tasks[q + NW];
-        grid_task<T, BWD>(a, fa, ma, cur, front, vec, pool);
+    GridTask t0 = a.tasks[q], t1 = a.tasks[q + NW];
+    TaskPre p0 = load_pre<BWD>(a, t0);
+    for (;;) {
+        TaskPre const p1 = load_pre<BWD>(a, t1);
+        GridTask const t2 = a.tasks[q + 2 * NW];
+        grid_task<T, BWD>(a, fa, ma, g, t0, p0, front, vec, pool);
+        q += NW;
+        if (q >= qe) break;
+        TaskPre const p2 = load_pre<BWD>(a, t2);
+        GridTask const t3 = a.tasks[q + 2 * NW];
+        grid_task<T, BWD>(a, fa, ma, g, t1, p1, front, vec, pool);
+        q += NW;
+        if (q >= qe) break;
+        t0 = t2;
+        t1 = t3;
+        p0 = p2;
     }
 }
 
@@ -516,7 +594,7 @@ __global__ void __launch_bounds__(NW* kWave) k_nr_grid(FusedArgs a, MisArgs ma, 
     int const g = a.order[a.order_off + blockIdx.x];
     int const b0 = a.bus_off[g], b1 = a.bus_off[g + 1];
     int const e0 = a.lvl_beg[g], e1 = a.lvl_beg[g + 1];
-    int const tid = threadIdx.x, wave = tid / kWave;
+    int const tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     T* pool = reinterpret_cast<T*>(grid_smem) + wave * a.pool;
     for (int it = 0; it < a.max_it; ++it) {
         double m = 0.0;
@@ -535,11 +613,11 @@ __global__ void __launch_bounds__(NW* kWave) k_nr_grid(FusedArgs a, MisArgs ma, 
             break;
         }
         for (int e = e0; e < e1; ++e) {
-            run_level<T, false, NW>(a, fa, ma, a.lvl_ptr[e] + wave, a.lvl_ptr[e + 1], front, vec, pool);
+            run_level<T, false, NW>(a, fa, ma, g, a.lvl_ptr[e] + wave, a.lvl_ptr[e + 1], front, vec, pool);
             __syncthreads();
         }
         for (int e = e1 - 1; e >= e0; --e) {
-            run_level<T, true, NW>(a, fa, ma, a.lvl_ptr[e] + wave, a.lvl_ptr[e + 1], front, vec, pool);
+            run_level<T, true, NW>(a, fa, ma, g, a.lvl_ptr[e] + wave, a.lvl_ptr[e + 1], front, vec, pool);
             __syncthreads();
         }
         bool const zero = a.lu_fail[g] != 0;
