@@ -1557,6 +1557,10 @@ int gaphip_plan_create(int n_inst, const gaphip_net* nets, int precision, gaphip
         *out = nullptr;
         auto p = std::make_unique<gaphip_plan>();
         build_plan(*p, n_inst, nets, precision);
+        // default launch shape (measured on the 4,096-grid batch): with FP32
+        // factors the biggest networks get 4 wavefronts; the FP64 kernel holds
+        // more registers per wavefront and runs best with 2 throughout
+        if (precision == GAPHIP_MIXED) p->tuning = gaphip_tuning{2, 4, 20000};
         *out = p.release();
     })
 }

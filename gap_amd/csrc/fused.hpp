@@ -301,18 +301,6 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     // gathers, unconditional: every list entry (the next task's and padding
     // included) holds valid indices; which entries are this task's is applied
     // where the values are used
-    double2 vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
-#pragma unroll
-    for (int k = 0; k < kPairChunks; ++k) {
-        vi[k] = ma.v[pr[k].x];
-        vj[k] = ma.v[pr[k].y];
-        si[k] = ma.s[pr[k].x];
-    }
-    T exv[kExtChunks];
-#pragma unroll
-    for (int k = 0; k < kExtChunks; ++k) exv[k] = ld_stream(front + ex[k].x);
-    double const fv = ma.f[rh.x];
-    T const vxv = vec[vv.x];
     bool pok[kPairChunks], eok[kExtChunks];
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) pok[k] = tk.pair_beg + k * kWave + lane < tk.pair_end;
@@ -320,6 +308,23 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     for (int k = 0; k < kExtChunks; ++k) eok[k] = tk.ext_beg + k * kWave + lane < tk.ext_end && ex[k].y >= 0;
     bool const rok = tk.rhs_beg + lane < tk.rhs_end && rh.y >= 0;
     bool const vok = tk.vx_beg + lane < tk.vx_end && vv.y >= 0;
+    // a lane whose value is not used reads lane 0's element instead, so its
+    // load falls into a line the wavefront fetches anyway
+    double2 vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
+#pragma unroll
+    for (int k = 0; k < kPairChunks; ++k) {
+        int const i0 = __builtin_amdgcn_readfirstlane(pr[k].x);
+        bool const diag = pr[k].x == pr[k].y;
+        vi[k] = ma.v[pok[k] ? pr[k].x : i0];
+        vj[k] = ma.v[pok[k] && !diag ? pr[k].y : i0];
+        si[k] = ma.s[pok[k] && diag ? pr[k].x : i0];
+    }
+    T exv[kExtChunks];
+#pragma unroll
+    for (int k = 0; k < kExtChunks; ++k)
+        exv[k] = ld_stream(front + (eok[k] ? ex[k].x : __builtin_amdgcn_readfirstlane(ex[k].x)));
+    double const fv = ma.f[rok ? rh.x : __builtin_amdgcn_readfirstlane(rh.x)];
+    T const vxv = vec[vok ? vv.x : __builtin_amdgcn_readfirstlane(vv.x)];
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
     wave_sync();  // the previous task of this wavefront has finished with the region
@@ -442,12 +447,12 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
     bool const sok = tk.sol_beg + lane < tk.sol_end && so.y >= 0;
     bool const act = gi < (tk.cls_cnt & 0xff);
     int const m = act ? pre.gf.z & 0xffff : 0, p = act ? pre.gf.z >> 16 : 0;
-    // unconditional loads: front records of inactive lanes are another
-    // front's or padding, and vectors, slot columns and front storage extend
-    // one 64-slot front past their end
-    T const sv = static_cast<T>(sol[so.x]);
-    T const wv = vec[pre.gf.y + t];
-    int const colt = fa.slot_col[pre.gf.w + t];
+    // unconditional loads: the front record of an inactive lane is another
+    // front's or padding
+    // (a lane whose value is not used reads lane 0's element: same line)
+    T const sv = static_cast<T>(sol[sok ? so.x : __builtin_amdgcn_readfirstlane(so.x)]);
+    T const wv = vec[t < p ? pre.gf.y + t : __builtin_amdgcn_readfirstlane(pre.gf.y)];
+    int const colt = fa.slot_col[t < p ? pre.gf.w + t : __builtin_amdgcn_readfirstlane(pre.gf.w)];
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
     wave_sync();  // the previous task of this wavefront has finished with the region
@@ -462,11 +467,13 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
         // lane t < p holds row t of U; x of the structure slots and the steps'
         // x_k come from the owning lanes by shuffles
         T urow[MM];
-        const T* Gr = front + pre.gf.x + static_cast<long long>(t) * (pre.gf.z & 0xffff);
+        int const g0 = __builtin_amdgcn_readfirstlane(pre.gf.x);
+        int const rbeg = pre.gf.x + t * m;
 #pragma unroll
         for (int q = 0; q < MM; ++q) {
-            T const x = ld_stream(Gr + q);
-            urow[q] = (t < p && q < m) ? x : T(0);
+            bool const use = t < p && q < m;
+            T const x = ld_stream(front + (use ? rbeg + q : g0));
+            urow[q] = use ? x : T(0);
         }
         wave_sync();
         T acc = t < m ? w[t] : T(0);
