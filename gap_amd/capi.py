@@ -91,6 +91,7 @@ SIGNATURES = {
     "gaphip_jacobian": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_double, _i, _i, _i, _d, C.c_int, _i]),
     "gaphip_mismatches": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_double, _d, C.c_int, _i]),
     "gaphip_adjoint": (C.c_int, [C.c_void_p, C.c_double, C.c_double, _d, _d, _d, _d, _d, _d, _d]),
+    "gaphip_branch_flows": (C.c_int, [C.c_void_p, _d]),
     "gaphip_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "gaphip_profile_get": (C.c_int, [C.c_void_p, C.c_int, _d, _ll, _d]),
     "gaphip_kernel_name": (C.c_char_p, [C.c_int]),
@@ -101,6 +102,7 @@ SIGNATURES = {
     "gaphip_lu_solve": (C.c_int, [C.c_void_p, _d, _d, C.c_int]),
     "gaphip_lu_is_factorized": (C.c_int, [C.c_void_p]),
 }
+FLOW_FIELDS = ("p_from", "p_to", "q_from", "q_to", "s_from", "s_to", "i_from", "i_from_angle", "i_to", "i_to_angle")
 PLUGIN_SYMBOLS = ("create_gpu_admittance_matrix", "create_gpu_lu_solver", "create_gpu_powerflow_solver")
 
 _lib = None
@@ -316,6 +318,16 @@ class Plan:
                                     None if gva is None else _p(gva, _d), _p(gp, _d), _p(gq, _d),
                                     _p(gr, _d), _p(gx, _d), _p(loss, _d)), "adjoint")
         return {"grad_p": gp, "grad_q": gq, "grad_r": gr[:nbr], "grad_x": gx[:nbr], "loss": loss}
+
+    def branch_flows(self) -> dict:
+        """Flows of every branch at the last solution (instance-concatenated
+        branch order): the BranchData result fields p_from, p_to, q_from, q_to
+        (W, VAr), s_from, s_to (VA), i_from, i_from_angle, i_to, i_to_angle
+        (A, rad); out-of-service branches are 0."""
+        nbr = sum(n.n_branch for n in self.nets)
+        out = np.zeros((max(nbr, 1), FLOW_FIELDS.__len__()))
+        _check(lib().gaphip_branch_flows(self._h, _p(out, _d)), "branch_flows")
+        return {name: out[:nbr, k].copy() for k, name in enumerate(FLOW_FIELDS)}
 
     def profile(self, on: bool = True):
         _check(lib().gaphip_profile_enable(self._h, int(on)), "profile_enable")

@@ -9,8 +9,9 @@ base S_base is 1 MVA there, as in the reference harness).  --synthetic builds
 one of the seeded workloads: the 200-bus grid (tree + 40 extra edges), the
 10k-bus meshed grid with transformers, shunts and PV buses, the 100k-bus
 meshed grid, or the batch of 4,096 grids of 50-5000 buses.  The output lists,
-per network, the iteration count, convergence and final mismatch, and per bus
-(single networks) |V| in pu and the angle in rad.
+per network, the iteration count, convergence and final mismatch, and for a
+single network per bus |V| in pu and the angle in rad and per branch the
+flows (p, q, s, current magnitude and angle at both ends).
 """
 from __future__ import annotations
 
@@ -93,6 +94,9 @@ def main(argv=None) -> int:
             if len(nets) == 1:
                 entry["node"] = [{"id": int(i), "u_pu": float(abs(x)), "u_angle": float(np.angle(x))}
                                  for i, x in zip(net.bus_id, v)]
+                fl = plan.branch_flows()
+                entry["line"] = [{"id": int(bid), **{f: float(fl[f][k]) for f in capi.FLOW_FIELDS}}
+                                 for k, bid in enumerate(net.br_id)]
             out["networks"].append(entry)
         with open(args.output, "w") as f:
             json.dump(out, f)

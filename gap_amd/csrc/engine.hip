@@ -877,6 +877,7 @@ struct gaphip_plan {
     gaphip::DBuf<double> flat, br_r, br_x, br_g, br_b, app_g, app_b, zb, f, dx, final_mis, jv64;
     gaphip::DBuf<float> jv32;
     gaphip::DBuf<int> br_f, br_t, br_inst;  // in-service branch endpoints (global bus) or -1
+    gaphip::DBuf<double> flow_zb, flow_ib, flow_out;  // branch flows (per instance bases, results)
     gaphip::DBuf<int> bus_type_d;
     gaphip::DBuf<double> adj_lossb, adj_gp, adj_gq, adj_gr, adj_gx, adj_gvm, adj_gva;
     gaphip::DBuf<double2> v, vpol, s, sspec, y_pu, y_given;
@@ -1519,6 +1520,30 @@ void run_adjoint(gaphip_plan& P, double w_loss, double w_dev, double const* g_vm
     }
 }
 
+// Branch flows at the last solution, kFlowFields per branch into host `out`.
+void run_branch_flows(gaphip_plan& P, double* out) {
+    if (!P.have_solution) throw Error(GAPHIP_E_STATE, "branch flows need a solve first");
+    if (!out) throw Error(GAPHIP_E_ARG, "null output");
+    if (P.n_br == 0) return;
+    hipStream_t s = P.stream;
+    int const G = P.n_inst;
+    std::vector<double> zb(G), ib(G);
+    for (int g = 0; g < G; ++g) {
+        double const ub = P.nets[g].u_rated[0];
+        zb[g] = ub * ub / P.last_base;
+        ib[g] = P.last_base / (std::sqrt(3.0) * ub);
+    }
+    P.flow_zb.upload(zb, s);
+    P.flow_ib.upload(ib, s);
+    P.flow_out.alloc(static_cast<size_t>(kFlowFields) * P.n_br);
+    FlowArgs fa{static_cast<int>(P.n_br), P.br_f.p, P.br_t.p, P.br_inst.p, P.br_r.p, P.br_x.p, P.br_g.p, P.br_b.p,
+                P.flow_zb.p, P.flow_ib.p, P.v.p, P.last_base, P.flow_out.p};
+    hipLaunchKernelGGL(k_branch_flow, dim3(blocks_for(P.n_br, 256)), dim3(256), 0, s, fa);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, P.flow_out.p, sizeof(double) * kFlowFields * P.n_br, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
 }  // namespace gaphip
 
 using namespace gaphip;
@@ -2053,6 +2078,13 @@ int gaphip_lu_solve(gaphip_lu* L, const double* rhs_ri, double* x_ri, int n) {
 }
 
 }  // extern "C"
+
+extern "C" int gaphip_branch_flows(gaphip_plan* P, double* out) {
+    GAPHIP_GUARD({
+        if (!P) throw Error(GAPHIP_E_ARG, "null plan");
+        run_branch_flows(*P, out);
+    })
+}
 
 extern "C" int gaphip_adjoint(gaphip_plan* P, double w_loss, double w_dev, const double* g_vm, const double* g_va,
                               double* grad_p, double* grad_q, double* grad_r, double* grad_x, double* loss_out) {

@@ -1085,6 +1085,71 @@ __global__ void __launch_bounds__(256) k_adj_branch(AdjBrArgs a) {
     a.gx[k] = gx;
 }
 
+// ------------------------------------------------------------ branch flows
+// Flows of every branch at a solution: the fields BranchData declares
+// (include/gap/core/types.h:233-243: p/q/s at both ends, current magnitude and
+// angle at both ends) and the reference never fills.  The branch's own 2x2
+// block of the Y-bus builder (cpu_admittance_matrix.cpp:121-135: series
+// y = 1/(r + jx) when |z|^2 > 1e-12, (y + g1) + j b1/2 on each diagonal, -y off
+// it), per unit with the network's impedance base; S = V conj(I) in VA and
+// currents in A of the network's voltage base, I_base = S_base / (sqrt(3) u_base).
+// Out-of-service branches (bf < 0) get zeros.  One thread per branch.
+constexpr int kFlowFields = 10;  // p_from, p_to, q_from, q_to, s_from, s_to, i_from, i_from_angle, i_to, i_to_angle
+struct FlowArgs {
+    int n_br;
+    const int* bf;
+    const int* bt;
+    const int* br_inst;
+    const double* r;
+    const double* x;
+    const double* g;
+    const double* b;
+    const double* zb;     // per instance: u_base^2 / S_base
+    const double* ibase;  // per instance: S_base / (sqrt(3) u_base)
+    const double2* v;
+    double base;
+    double* out;          // kFlowFields per branch
+};
+__global__ void __launch_bounds__(256) k_branch_flow(FlowArgs a) {
+#pragma clang fp contract(off)
+    int const k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n_br) return;
+    double* o = a.out + static_cast<long long>(kFlowFields) * k;
+    int const f = a.bf[k], t = a.bt[k];
+    if (f < 0) {
+        for (int q = 0; q < kFlowFields; ++q) o[q] = 0.0;
+        return;
+    }
+    double const r = a.r[k], x = a.x[k], z2 = r * r + x * x;
+    double yr = 0.0, yi = 0.0;
+    if (z2 > 1e-12) {
+        yr = r / z2;
+        yi = -x / z2;
+    }
+    int const g = a.br_inst[k];
+    double const zb = a.zb[g];
+    double const dr = (yr + a.g[k]) * zb, di = (yi + a.b[k] / 2.0) * zb;  // diagonal, per unit
+    double const orr = -yr * zb, oi = -yi * zb;                          // off-diagonal
+    double2 const vf = a.v[f], vt = a.v[t];
+    double const ifr = (dr * vf.x - di * vf.y) + (orr * vt.x - oi * vt.y);
+    double const ifi = (dr * vf.y + di * vf.x) + (orr * vt.y + oi * vt.x);
+    double const itr = (dr * vt.x - di * vt.y) + (orr * vf.x - oi * vf.y);
+    double const iti = (dr * vt.y + di * vt.x) + (orr * vf.y + oi * vf.x);
+    double const pf = (vf.x * ifr + vf.y * ifi) * a.base, qf = (vf.y * ifr - vf.x * ifi) * a.base;
+    double const pt = (vt.x * itr + vt.y * iti) * a.base, qt = (vt.y * itr - vt.x * iti) * a.base;
+    double const ib = a.ibase[g];
+    o[0] = pf;
+    o[1] = pt;
+    o[2] = qf;
+    o[3] = qt;
+    o[4] = sqrt(pf * pf + qf * qf);
+    o[5] = sqrt(pt * pt + qt * qt);
+    o[6] = sqrt(ifr * ifr + ifi * ifi) * ib;
+    o[7] = atan2(ifi, ifr);
+    o[8] = sqrt(itr * itr + iti * iti) * ib;
+    o[9] = atan2(iti, itr);
+}
+
 // ------------------------------------------------------------ update
 // Damped, clamped polar update of cpu_newton_raphson.cpp:438-504, one thread
 // per bus: angle step (0.9, |d| <= 0.15) for every bus but the first slack,

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define GAPHIP_ABI_VERSION 2
+#define GAPHIP_ABI_VERSION 3
 
 enum {
     GAPHIP_OK = 0,
@@ -192,6 +192,20 @@ int gaphip_mismatches(gaphip_plan* plan, int inst, const double* v_ri, double ba
 int gaphip_adjoint(gaphip_plan* plan, double w_loss, double w_dev, const double* g_vm,
                    const double* g_va, double* grad_p, double* grad_q, double* grad_r,
                    double* grad_x, double* loss_out);
+
+/* Branch flows at the last solution of every instance: per branch, in
+ * instance-concatenated branch order, the ten BranchData result fields
+ * (include/gap/core/types.h:233-243, which the reference declares but never
+ * fills) in struct order: p_from, p_to, q_from, q_to (W, VAr), s_from, s_to
+ * (VA), i_from, i_from_angle, i_to, i_to_angle (A, rad).  Computed from the
+ * branch's own block of the Y-bus (series 1/(r1 + j x1), g1 + j b1/2 at each
+ * end, the builder's model, cpu_admittance_matrix.cpp:121-135) in the
+ * network's per-unit system; currents in A of its voltage base u_rated[0].
+ * Out-of-service branches get zeros; `loading` needs a rating BranchData does
+ * not carry and is not computed.  out: 10 * total branches doubles.
+ * Before any solve: GAPHIP_E_STATE. */
+#define GAPHIP_FLOW_FIELDS 10
+int gaphip_branch_flows(gaphip_plan* plan, double* out);
 
 /* Timing hook for measurement: after each solve, the summed device time and
  * launch count of each kernel family, and the algorithmic HBM bytes they move.

@@ -25,3 +25,10 @@ def test_cli_synthetic_200_matches_capi(device, tmp_path):
     assert doc["networks"][0]["iterations"] == int(ref["iterations"][0])
     u = np.array([n["u_pu"] for n in node])
     assert np.abs(u - np.abs(ref["v"][0])).max() < 1e-12
+    plan = capi.Plan([net])
+    plan.solve(1e-8, 100, 100e6)
+    fl = plan.branch_flows()
+    line = doc["networks"][0]["line"]
+    assert [row["id"] for row in line] == [int(b) for b in net.br_id]
+    for f in capi.FLOW_FIELDS:
+        assert np.array_equal(np.array([row[f] for row in line]), fl[f])
