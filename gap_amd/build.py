@@ -59,8 +59,11 @@ def build(force: bool = True) -> None:
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", core, eng_o, sym_o,
           "-Wl,-soname,libgap_hip.so"])
     plugins = LIB / "libgap_hip_plugins.so"
-    _run(["g++", *CXX_FLAGS, "-shared", "-I", INC, SRC / "backends.cpp", "-o", plugins,
+    _run(["g++", *CXX_FLAGS, "-shared", "-I", INC, SRC / "backends.cpp", SRC / "pgm_io.cpp", "-o", plugins,
           "-L", LIB, "-lgap_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-soname,libgap_hip_plugins.so"])
+    # command-line front end (PGM JSON in and out)
+    _run(["g++", *CXX_FLAGS, "-I", INC, SRC / "gap_hip_main.cpp", "-o", LIB / "gap_hip",
+          "-L", LIB, "-lgap_hip_plugins", "-lgap_hip", "-Wl,-rpath,$ORIGIN"])
     for alias in ("libgap_admittance_gpu.so", "libgap_lu_gpu.so", "libgap_powerflow_gpu.so"):
         dst = LIB / alias
         if dst.exists() or dst.is_symlink():

@@ -249,6 +249,7 @@ void HipNewtonRaphson::give_y(SparseMatrix const& y, int per_unit) {
 solver::PowerFlowResult HipNewtonRaphson::solve_power_flow(NetworkData const& network_data,
                                                            SparseMatrix const& admittance_matrix,
                                                            solver::PowerFlowConfig const& config) {
+    flows_valid_ = false;
     plan_for(network_data);
     give_y(admittance_matrix, 0);
     gaphip_solve_opts o{config.tolerance, config.max_iterations, config.base_power};
@@ -258,6 +259,7 @@ solver::PowerFlowResult HipNewtonRaphson::solve_power_flow(NetworkData const& ne
     double fm = 0.0;
     int rc = gaphip_solve(plan_, &o, v.data(), &its, &conv, &fm);
     if (rc) raise(rc, "solve_power_flow");
+    flows_valid_ = true;
     solver::PowerFlowResult r;
     r.converged = conv != 0;
     r.iterations = its;
@@ -280,12 +282,38 @@ solver::PowerFlowResult HipNewtonRaphson::solve_power_flow(NetworkData const& ne
     return r;
 }
 
+std::vector<BranchData> HipNewtonRaphson::branch_flows(NetworkData const& network_data) {
+    if (!flows_valid_ || !plan_ || !cached_ || !cached_->same_topology(NetArrays(network_data)))
+        throw std::runtime_error("branch_flows: no solve_power_flow of this network precedes it");
+    size_t const nb = network_data.branches.size();
+    std::vector<double> f(GAPHIP_FLOW_FIELDS * std::max<size_t>(nb, 1));
+    int rc = gaphip_branch_flows(plan_, f.data());
+    if (rc) raise(rc, "branch_flows");
+    std::vector<BranchData> out(network_data.branches);
+    for (size_t k = 0; k < nb; ++k) {
+        double const* r = f.data() + GAPHIP_FLOW_FIELDS * k;
+        BranchData& b = out[k];
+        b.p_from = r[0];
+        b.p_to = r[1];
+        b.q_from = r[2];
+        b.q_to = r[3];
+        b.s_from = r[4];
+        b.s_to = r[5];
+        b.i_from = r[6];
+        b.i_from_angle = r[7];
+        b.i_to = r[8];
+        b.i_to_angle = r[9];
+    }
+    return out;
+}
+
 std::vector<Float> HipNewtonRaphson::calculate_mismatches(NetworkData const& network_data,
                                                           ComplexVector const& bus_voltages,
                                                           SparseMatrix const& admittance_matrix) {
     // public entry of the reference: S_base fixed at 100 MVA, Y used as given,
     // result in watts (cpu_newton_raphson.cpp:225-240)
     double const base = 100e6;
+    flows_valid_ = false;  // the mismatch evaluation replaces the plan's voltages
     plan_for(network_data);
     int const n = static_cast<int>(network_data.buses.size());
     if (static_cast<int>(bus_voltages.size()) != n) throw std::runtime_error("voltage vector size mismatch");

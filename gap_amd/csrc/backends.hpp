@@ -64,6 +64,13 @@ class HipNewtonRaphson final : public solver::IPowerFlowSolver {
     std::vector<solver::IterationState> const& get_iteration_states() const override { return states_; }
     void clear_iteration_states() override { states_.clear(); }
 
+    // Copies of network_data's branches with the BranchData result fields
+    // (p/q/s/i at both ends) filled at the solution of the last
+    // solve_power_flow on that network (types.h:233-243; the reference never
+    // fills them).  No counterpart in the reference.  Throws when no solve of
+    // this network precedes it or another call has used the plan since.
+    std::vector<BranchData> branch_flows(NetworkData const& network_data);
+
   private:
     gaphip_plan* plan_for(NetworkData const& net);
     void give_y(SparseMatrix const& y, int per_unit);
@@ -72,6 +79,7 @@ class HipNewtonRaphson final : public solver::IPowerFlowSolver {
     std::unique_ptr<NetArrays> cached_;
     std::shared_ptr<solver::ILUSolver> lu_;
     bool capture_ = false;
+    bool flows_valid_ = false;  // the plan holds the last solve's solution
     std::vector<solver::IterationState> states_;
 };
 

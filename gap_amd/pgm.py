@@ -102,7 +102,10 @@ def rows_from_pgm(data: dict, base_power: float = 1e6):
 
 
 def read_pgm_output_voltages(path):
-    """|V| (pu) and angle (rad) of the PGM output dataset, ordered by node id."""
+    """|V| (pu) and the angle of the PGM output dataset as stored, ordered by
+    node id.  The reference's fixtures hold the angle in degrees (its
+    tests/pgm_validation/run_validation.py:223-226 reads them so; a 2-bus check
+    of the angle drop (P x - Q r) / U^2 confirms it)."""
     with open(path) as f:
         doc = json.load(f)
     nodes = sorted(doc["node"], key=lambda n: n["id"])
