@@ -303,6 +303,12 @@ std::vector<BranchData> HipNewtonRaphson::branch_flows(NetworkData const& networ
         b.i_from_angle = r[7];
         b.i_to = r[8];
         b.i_to_angle = r[9];
+        // power-grid-model's definition: current over rated current for a
+        // line, apparent power over rated power for a transformer
+        if (b.branch_type == BranchType::TRAFO)
+            b.loading = b.sn > 0.0 ? std::max(b.s_from, b.s_to) / b.sn : 0.0;
+        else
+            b.loading = b.i_n > 0.0 ? std::max(b.i_from, b.i_to) / b.i_n : 0.0;
     }
     return out;
 }

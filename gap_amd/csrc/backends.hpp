@@ -65,7 +65,8 @@ class HipNewtonRaphson final : public solver::IPowerFlowSolver {
     void clear_iteration_states() override { states_.clear(); }
 
     // Copies of network_data's branches with the BranchData result fields
-    // (p/q/s/i at both ends) filled at the solution of the last
+    // (p/q/s/i at both ends, and loading: current over i_n for a line, apparent
+    // power over sn for a transformer) filled at the solution of the last
     // solve_power_flow on that network (types.h:233-243; the reference never
     // fills them).  No counterpart in the reference.  Throws when no solve of
     // this network precedes it or another call has used the plan since.

@@ -323,9 +323,11 @@ Rows rows_from_pgm(Json const& doc, double base_power) {
         double const r_pu = number(l, "r1", "line") / z_max;
         double const x_pu = number(l, "x1", "line") / z_max;
         double const b_pu = omega * number_or(l, "c1", 0.0, "line") * z_max;
-        if (status)
+        if (status) {
             r.branch.push_back({static_cast<double>(id), static_cast<double>(f), static_cast<double>(t), r_pu * z0,
                                 x_pu * z0, b_pu / z0});
+            r.branch_i_n.push_back(number_or(l, "i_n", 0.0, "line"));
+        }
     }
     return r;
 }
@@ -356,6 +358,7 @@ NetworkData network_from_rows(Rows const& rows) {
         br.r1 = row[3];
         br.x1 = row[4];
         br.b1 = row.size() > 5 ? row[5] : 0.0;
+        if (net.branches.size() < rows.branch_i_n.size()) br.i_n = rows.branch_i_n[net.branches.size()];
         net.branches.push_back(br);
     }
     net.num_buses = static_cast<int>(net.buses.size());
@@ -452,7 +455,7 @@ void PgmJsonIO::write_output(std::string const& filename, NetworkData const& net
         out += (k ? "," : "") + std::string("{\"id\":") + std::to_string(b.id) + ",\"energized\":" +
                std::to_string(b.status ? 1 : 0) + ",\"p_from\":" + fmt(b.p_from) + ",\"q_from\":" + fmt(b.q_from) +
                ",\"i_from\":" + fmt(b.i_from) + ",\"s_from\":" + fmt(b.s_from) + ",\"p_to\":" + fmt(b.p_to) +
-               ",\"q_to\":" + fmt(b.q_to) + ",\"i_to\":" + fmt(b.i_to) + ",\"s_to\":" + fmt(b.s_to) + "}";
+               ",\"q_to\":" + fmt(b.q_to) + ",\"i_to\":" + fmt(b.i_to) + ",\"s_to\":" + fmt(b.s_to) + ",\"loading\":" + fmt(b.loading) + "}";
     }
     out += "]}," + meta(converged, iterations, final_mismatch, base_power_) + "}\n";
     write_text(filename, out);

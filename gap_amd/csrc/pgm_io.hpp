@@ -48,11 +48,13 @@ Json parse_json(std::string const& text);
 // [id, from, to, r, x, b] as the harness passes them to solve_simple_power_flow.
 struct Rows {
     std::vector<std::vector<double>> bus, branch;
+    std::vector<double> branch_i_n;  // rated current of each branch row's line (PGM i_n, A; 0 if absent)
 };
 Rows rows_from_pgm(Json const& doc, double base_power);
 
 // NetworkData from rows, as solve_simple_power_flow converts them
-// (bindings/solver_bindings.cpp:64-118).
+// (bindings/solver_bindings.cpp:64-118); a line's rated current, when the rows
+// carry it, goes to BranchData::i_n (it does not enter the solve).
 NetworkData network_from_rows(Rows const& rows);
 
 // Number formatting that round-trips a double.

@@ -12,7 +12,7 @@ from pathlib import Path
 
 REF = Path("/root/reference/tests/pgm_validation/test_data")
 OUT = Path(__file__).resolve().parent / "pgm_lines.json"
-FIELDS = ("p_from", "q_from", "i_from", "s_from", "p_to", "q_to", "i_to", "s_to")
+FIELDS = ("p_from", "q_from", "i_from", "s_from", "p_to", "q_to", "i_to", "s_to", "loading")
 
 
 def main():
