@@ -92,6 +92,8 @@ SIGNATURES = {
     "gaphip_mismatches": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_double, _d, C.c_int, _i]),
     "gaphip_adjoint": (C.c_int, [C.c_void_p, C.c_double, C.c_double, _d, _d, _d, _d, _d, _d, _d]),
     "gaphip_branch_flows": (C.c_int, [C.c_void_p, _d]),
+    "gaphip_plan_update_branches": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i, _d, _d]),
+    "gaphip_plan_get_ybus": (C.c_int, [C.c_void_p, C.c_int, _i, _i, _d, C.c_int, C.POINTER(C.c_int)]),
     "gaphip_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "gaphip_profile_get": (C.c_int, [C.c_void_p, C.c_int, _d, _ll, _d]),
     "gaphip_kernel_name": (C.c_char_p, [C.c_int]),
@@ -318,6 +320,35 @@ class Plan:
                                     None if gva is None else _p(gva, _d), _p(gp, _d), _p(gq, _d),
                                     _p(gr, _d), _p(gx, _d), _p(loss, _d)), "adjoint")
         return {"grad_p": gp, "grad_q": gq, "grad_r": gr[:nbr], "grad_x": gx[:nbr], "loss": loss}
+
+    def update_branches(self, inst: int, branch, r1, x1):
+        """New r1 / x1 (ohm) for branches `branch` of instance `inst`; once the
+        plan's Y-bus is assembled (by a solve), the Y slots they touch are
+        recomputed on the device, bit-identical to a fresh assembly."""
+        b = np.ascontiguousarray(branch, np.int32)
+        r = np.ascontiguousarray(r1, np.float64)
+        x = np.ascontiguousarray(x1, np.float64)
+        if not (len(b) == len(r) == len(x)):
+            raise ValueError("branch, r1 and x1 differ in length")
+        _check(lib().gaphip_plan_update_branches(self._h, inst, len(b), _p(b, _i), _p(r, _d), _p(x, _d)),
+               "update_branches")
+        for k, bi in enumerate(b):
+            self.nets[inst].br_r[bi] = r[k]
+            self.nets[inst].br_x[bi] = x[k]
+
+    def ybus(self, inst: int = 0):
+        """(row_ptr, col_idx, values) of the plan's per-unit Y-bus of `inst`."""
+        nnz = C.c_int(0)
+        n = self.nets[inst].n_bus
+        rc = lib().gaphip_plan_get_ybus(self._h, inst, None, None, None, 0, C.byref(nnz))
+        if rc not in (OK, E_CAPACITY):
+            _check(rc, "get_ybus")
+        rp = np.zeros(n + 1, np.int32)
+        ci = np.zeros(max(nnz.value, 1), np.int32)
+        val = np.zeros(2 * max(nnz.value, 1))
+        _check(lib().gaphip_plan_get_ybus(self._h, inst, _p(rp, _i), _p(ci, _i), _p(val, _d), nnz.value,
+                                          C.byref(nnz)), "get_ybus")
+        return rp, ci[:nnz.value], val[0:2 * nnz.value:2] + 1j * val[1:2 * nnz.value:2]
 
     def branch_flows(self) -> dict:
         """Flows of every branch at the last solution (instance-concatenated

@@ -900,6 +900,10 @@ struct gaphip_plan {
     int maxf_rows = 0;
     std::vector<unsigned long long> maxf_host;
     bool have_solution = false;
+    bool y_assembled = false;  // y_pu holds the Y-bus of the last solve's base
+    std::vector<std::vector<int>> br_slot_ptr, br_slot;  // per instance: Y slots each branch touches (lazy)
+    gaphip::DBuf<int> upd_idx, upd_slots;
+    gaphip::DBuf<double> upd_r, upd_x;
     double last_base = 100e6;
     gaphip::Profiler prof;
     int* pinned_active = nullptr;
@@ -1280,6 +1284,7 @@ void run_solve(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front, T* v
                 P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.y_pu.p, nullptr};
     launch(P, K_YBUS, 48.0 * Z, [&] { hipLaunchKernelGGL(k_ybus, dim3(blocks_for(Z, 256)), dim3(256), 0, s, ya); });
     apply_given_y(P, o.base_power);
+    P.y_assembled = true;
     FlatArgs fa{static_cast<int>(N), P.flat.p, P.v.p};
     launch(P, K_FLAT, 24.0 * N, [&] { hipLaunchKernelGGL(k_flat, dim3(blocks_for(N, 256)), dim3(256), 0, s, fa); });
 
@@ -1380,6 +1385,7 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
                 P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.y_pu.p, nullptr};
     launch(P, K_YBUS, 48.0 * Z, [&] { hipLaunchKernelGGL(k_ybus, dim3(blocks_for(Z, 256)), dim3(256), 0, s, ya); });
     apply_given_y(P, o.base_power);
+    P.y_assembled = true;
     FlatArgs fa{static_cast<int>(N), P.flat.p, P.v.p};
     launch(P, K_FLAT, 24.0 * N, [&] { hipLaunchKernelGGL(k_flat, dim3(blocks_for(N, 256)), dim3(256), 0, s, fa); });
 
@@ -1518,6 +1524,78 @@ void run_adjoint(gaphip_plan& P, double w_loss, double w_dev, double const* g_vm
             loss_out[g] = acc;
         }
     }
+}
+
+// New r1 / x1 for n branches of instance g, applied to the plan's branch
+// arrays and, when its Y-bus is assembled, to exactly the Y slots those
+// branches contribute to (same per-slot routine as the full assembly).
+void run_update_branches(gaphip_plan& P, int g, int n, const int* branch, const double* r, const double* x) {
+    if (g < 0 || g >= P.n_inst || n < 0 || (n > 0 && (!branch || !r || !x))) throw Error(GAPHIP_E_ARG, "bad argument");
+    if (n == 0) return;
+    auto& N = P.nets[g];
+    int const m = N.n_branch();
+    for (int t = 0; t < n; ++t)
+        if (branch[t] < 0 || branch[t] >= m) throw Error(GAPHIP_E_ARG, "branch index out of range");
+    if (P.br_slot_ptr.empty()) {
+        P.br_slot_ptr.resize(P.n_inst);
+        P.br_slot.resize(P.n_inst);
+    }
+    if (P.br_slot_ptr[g].empty()) {
+        auto const& Y = P.yp[g];
+        std::vector<int> ptr(m + 1, 0);
+        int const slots = static_cast<int>(Y.ci.size());
+        for (int k = 0; k < slots; ++k)
+            for (int q = Y.seg[k]; q < Y.seg[k + 1]; ++q)
+                if ((Y.code[q] & 3) != kContribShunt) ptr[(Y.code[q] >> 2) + 1]++;
+        for (int b = 0; b < m; ++b) ptr[b + 1] += ptr[b];
+        std::vector<int> sl(ptr[m]), fill(ptr.begin(), ptr.end() - 1);
+        for (int k = 0; k < slots; ++k)
+            for (int q = Y.seg[k]; q < Y.seg[k + 1]; ++q)
+                if ((Y.code[q] & 3) != kContribShunt) sl[fill[Y.code[q] >> 2]++] = P.y_off[g] + k;
+        P.br_slot_ptr[g] = std::move(ptr);
+        P.br_slot[g] = std::move(sl);
+    }
+    std::vector<int> gidx(n), slots;
+    for (int t = 0; t < n; ++t) {
+        N.br_r[branch[t]] = r[t];
+        N.br_x[branch[t]] = x[t];
+        gidx[t] = static_cast<int>(P.br_off[g]) + branch[t];
+        for (int q = P.br_slot_ptr[g][branch[t]]; q < P.br_slot_ptr[g][branch[t] + 1]; ++q) slots.push_back(P.br_slot[g][q]);
+    }
+    std::sort(slots.begin(), slots.end());
+    slots.erase(std::unique(slots.begin(), slots.end()), slots.end());
+    hipStream_t s = P.stream;
+    std::vector<double> rv(r, r + n), xv(x, x + n);  // alive until the synchronisation below
+    P.upd_idx.upload(gidx, s);
+    P.upd_r.upload(rv, s);
+    P.upd_x.upload(xv, s);
+    hipLaunchKernelGGL(k_branch_rx_set, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, P.upd_idx.p, P.upd_r.p, P.upd_x.p,
+                       P.br_r.p, P.br_x.p);
+    bool const given = !P.y_given_mode.empty() && P.y_given_mode[g];
+    if (P.y_assembled && !given && !slots.empty()) {
+        P.upd_slots.upload(slots, s);
+        YbusArgs ya{static_cast<int>(P.y_nnz), P.y_seg.p, P.y_code.p, P.y_slot_inst.p, P.br_r.p, P.br_x.p, P.br_g.p,
+                    P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.y_pu.p, nullptr};
+        int const ns = static_cast<int>(slots.size());
+        hipLaunchKernelGGL(k_ybus_slots, dim3(blocks_for(ns, 256)), dim3(256), 0, s, ya, ns, P.upd_slots.p);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// The plan's Y-bus of instance g (per unit, instance-local CSR).
+void run_get_ybus(gaphip_plan const& P, int g, int* rp, int* ci, double* val_ri, int cap, int* nnz_out) {
+    if (g < 0 || g >= P.n_inst) throw Error(GAPHIP_E_ARG, "bad instance");
+    auto const& Y = P.yp[g];
+    int const nnz = static_cast<int>(Y.ci.size());
+    if (nnz_out) *nnz_out = nnz;
+    if (!P.y_assembled) throw Error(GAPHIP_E_STATE, "the plan's Y-bus is assembled by the first solve");
+    if (cap < nnz) throw Error(GAPHIP_E_CAPACITY, "output holds " + std::to_string(cap) + " entries, Y has " +
+                                                      std::to_string(nnz));
+    if (rp) std::copy(Y.rp.begin(), Y.rp.end(), rp);
+    if (ci) std::copy(Y.ci.begin(), Y.ci.end(), ci);
+    if (val_ri && nnz)
+        HIPCHK(hipMemcpy(val_ri, P.y_pu.p + P.y_off[g], sizeof(double2) * nnz, hipMemcpyDeviceToHost));
 }
 
 // Branch flows at the last solution, kFlowFields per branch into host `out`.
@@ -2078,6 +2156,22 @@ int gaphip_lu_solve(gaphip_lu* L, const double* rhs_ri, double* x_ri, int n) {
 }
 
 }  // extern "C"
+
+extern "C" int gaphip_plan_update_branches(gaphip_plan* P, int inst, int n, const int* branch, const double* r,
+                                           const double* x) {
+    GAPHIP_GUARD({
+        if (!P) throw Error(GAPHIP_E_ARG, "null plan");
+        run_update_branches(*P, inst, n, branch, r, x);
+    })
+}
+
+extern "C" int gaphip_plan_get_ybus(const gaphip_plan* P, int inst, int* row_ptr, int* col_idx, double* values_ri,
+                                    int cap, int* nnz) {
+    GAPHIP_GUARD({
+        if (!P) throw Error(GAPHIP_E_ARG, "null plan");
+        run_get_ybus(*P, inst, row_ptr, col_idx, values_ri, cap, nnz);
+    })
+}
 
 extern "C" int gaphip_branch_flows(gaphip_plan* P, double* out) {
     GAPHIP_GUARD({

@@ -32,10 +32,9 @@ struct YbusArgs {
     double2* y_si;     // optional: siemens values
 };
 
-__global__ void __launch_bounds__(256) k_ybus(YbusArgs a) {
+// One Y slot: its contributions in the builder's order (segment k of code).
+__device__ inline void ybus_slot(YbusArgs const& a, int k) {
 #pragma clang fp contract(off)
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.nnz) return;
     double re = 0.0, im = 0.0;
     for (int s = a.seg[k]; s < a.seg[k + 1]; ++s) {
         int c = a.code[s];
@@ -65,6 +64,26 @@ __global__ void __launch_bounds__(256) k_ybus(YbusArgs a) {
     if (a.y_si) a.y_si[k] = make_double2(re, im);
     double zb = a.zb[a.slot_inst[k]];
     a.y_pu[k] = make_double2(re * zb, im * zb);
+}
+
+__global__ void __launch_bounds__(256) k_ybus(YbusArgs a) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < a.nnz) ybus_slot(a, k);
+}
+
+// Incremental update: new r / x of n branches, then the listed slots (those
+// the branches touch) recomputed by the same per-slot routine, so the result
+// equals a full assembly of the changed network bit for bit.
+__global__ void __launch_bounds__(256) k_branch_rx_set(int n, const int* idx, const double* r, const double* x,
+                                                       double* br_r, double* br_x) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    br_r[idx[t]] = r[t];
+    br_x[idx[t]] = x[t];
+}
+__global__ void __launch_bounds__(256) k_ybus_slots(YbusArgs a, int n, const int* slots) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) ybus_slot(a, slots[t]);
 }
 
 // Caller-supplied Y-bus: y_pu = given * scale (scale = Z_base, or 1 when the

@@ -117,6 +117,25 @@ int gaphip_plan_stats_get(const gaphip_plan* plan, gaphip_plan_stats* out);
 int gaphip_plan_set_injections(gaphip_plan* plan, int inst, const double* p_w, const double* q_var);
 /* Replace branch r1 / x1 (ohm) of instance `inst` (length n_branch). */
 int gaphip_plan_set_branch_rx(gaphip_plan* plan, int inst, const double* r, const double* x);
+/* Incremental admittance update on the device (IAdmittanceMatrix::
+ * update_admittance_matrix, include/gap/admittance/admittance_interface.h:29-30,
+ * for the plan-resident Y-bus): new r1 / x1 (ohm) for `n` branches of instance
+ * `inst` (indices within the instance).  Once a solve has assembled the plan's
+ * Y-bus, exactly the Y slots those branches contribute to are recomputed on the
+ * device with the full assembly's summation order, so the result is
+ * bit-identical to assembling the changed network afresh (not the reference's
+ * delta update, which double-counts b1, cpu_admittance_matrix.cpp:247-266).
+ * Solves re-assemble the whole Y-bus as before.  Instances whose Y-bus the
+ * caller supplied (gaphip_plan_set_ybus) only get the new r1 / x1. */
+int gaphip_plan_update_branches(gaphip_plan* plan, int inst, int n, const int* branch, const double* r1,
+                                const double* x1);
+/* The plan's Y-bus of instance `inst`: per unit (siemens * u_rated[0]^2 / S_base
+ * of the last solve), instance-local CSR (row_ptr n_bus + 1, col_idx and
+ * values_ri (re, im pairs) of *nnz entries, which is always set).  Before the
+ * first solve GAPHIP_E_STATE; cap < nnz GAPHIP_E_CAPACITY.  Any output pointer
+ * may be NULL. */
+int gaphip_plan_get_ybus(const gaphip_plan* plan, int inst, int* row_ptr, int* col_idx, double* values_ri, int cap,
+                         int* nnz);
 /* Use a caller-supplied Y-bus for instance `inst` instead of assembling it
  * (IPowerFlowSolver::solve_power_flow receives Y from the caller,
  * powerflow_interface.h:59-61).  The pattern must equal the plan's Y-bus
