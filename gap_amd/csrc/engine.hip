@@ -685,15 +685,38 @@ struct FrontRunner {
     static void big_factor(DevFronts const& d, FrontArgs const& a, int beg, int nb, int mmax, int pmax,
                            T const* jv, T* front, hipStream_t s) {
         hipLaunchKernelGGL(k_big_assemble<T>, dim3(mmax, nb), dim3(256), 0, s, a, d.big(), beg, jv, front);
-        for (int k0 = 0; k0 < pmax; k0 += kPanel) {
-            hipLaunchKernelGGL(k_big_diag<T>, dim3(nb), dim3(256), 0, s, a, beg, k0, front);
-            int const rest = mmax - std::min(k0 + kPanel, pmax);
-            if (rest <= 0) continue;
-            hipLaunchKernelGGL(k_big_trsm_rows<T>, dim3(blocks_for(rest, 64), nb), dim3(256), 0, s, a, beg, k0, front);
-            hipLaunchKernelGGL(k_big_trsm<T>, dim3(blocks_for(rest, 256), nb), dim3(256), 0, s, a, beg, k0, kPanel,
-                               front);
-            int const tiles = blocks_for(rest, 128);
-            hipLaunchKernelGGL(k_big_gemm<T>, dim3(tiles, tiles, nb), dim3(256), 0, s, a, beg, k0, kPanel, front);
+        // Grids are sized for the group: a front's rows below its panel start
+        // at min(k0 + 64, p_f) > k0, so mmax - k0 rows (mmax - j columns right
+        // of an outer panel) bound every front; blocks past a front's own
+        // bounds return at once.
+        // Two-level panels only where the trailing matrix is large enough for
+        // its traffic to dominate (below kTwoLevelMin slots the extra row-block
+        // updates and solves cost more than they save): outer width 64 is the
+        // plain right-looking scheme.
+        int const outer = mmax >= kTwoLevelMin ? kBigOuter : kPanel;
+        for (int j = 0; j < pmax; j += outer) {
+            int const jlim = std::min(j + outer, pmax);
+            for (int k0 = j; k0 < jlim; k0 += kPanel) {
+                int const below = mmax - k0, inner = jlim - k0;
+                hipLaunchKernelGGL(k_big_diag<T>, dim3(nb), dim3(kWave), 0, s, a, beg, k0, front);
+                hipLaunchKernelGGL(k_big_trsm_rows<T>, dim3(blocks_for(below, 64), nb), dim3(256), 0, s, a, beg, k0,
+                                   front);
+                if (inner <= kPanel) continue;  // no front has columns left inside the outer panel
+                hipLaunchKernelGGL(k_big_trsm<T>, dim3(blocks_for(inner, 256), nb), dim3(256), 0, s, a, beg, k0,
+                                   kPanel, -1, j + outer, front);
+                hipLaunchKernelGGL(k_big_gemm<T>, dim3(blocks_for(inner, 128), blocks_for(below, 128), nb), dim3(256),
+                                   0, s, a, beg, k0, kPanel, j + outer, front);
+            }
+            int const right = mmax - j;
+            for (int k0 = j; k0 < jlim; k0 += kPanel) {
+                if (k0 > j)
+                    hipLaunchKernelGGL(k_big_gemm_rows<T>, dim3(blocks_for(right, 128), 1, nb), dim3(256), 0, s, a,
+                                       beg, j, outer, k0, front);
+                hipLaunchKernelGGL(k_big_trsm<T>, dim3(blocks_for(right, 256), nb), dim3(256), 0, s, a, beg, k0,
+                                   kPanel, j + outer, -1, front);
+            }
+            int const tiles = blocks_for(right, 128);
+            hipLaunchKernelGGL(k_big_gemm<T>, dim3(tiles, tiles, nb), dim3(256), 0, s, a, beg, j, outer, -1, front);
         }
     }
     // blocked forward / backward solves of the fronts [beg, beg+nb) of a large-front group
@@ -702,10 +725,9 @@ struct FrontRunner {
         hipLaunchKernelGGL(k_big_fwd_init<T>, dim3(nb), dim3(256), 0, s, a, beg, vec, rhs, sign);
         for (int k0 = 0; k0 < pmax; k0 += kSolvePanel) {
             hipLaunchKernelGGL(k_big_fwd_diag<T>, dim3(nb), dim3(kWave), 0, s, a, beg, k0, front, vec);
-            int const rest = mmax - std::min(k0 + kSolvePanel, pmax);
-            if (rest > 0)
-                hipLaunchKernelGGL(k_big_fwd_update<T>, dim3(blocks_for(rest, 256), nb), dim3(256), 0, s, a, beg, k0,
-                                   front, vec);
+            // rows below a front's panel start at min(k0 + 64, p_f) > k0: mmax - k0 bounds them all
+            hipLaunchKernelGGL(k_big_fwd_update<T>, dim3(blocks_for(mmax - k0, 256), nb), dim3(256), 0, s, a, beg, k0,
+                               front, vec);
         }
     }
     static void big_backward(DevFronts const&, FrontArgs const& a, int beg, int nb, int pmax, T const* front, T* vec,

@@ -480,13 +480,19 @@ __global__ void __launch_bounds__(256) k_big_assemble(FrontArgs a, BigMaps b, in
     }
 }
 
-// Panels of kBigPanel pivots.  For each: the diagonal block A11 is factored by
-// one workgroup per front (multipliers stored below its diagonal), the rows
-// below become L21 = A21 U11^-1 (k_big_trsm_rows), the columns to the right
-// U12 = L11^-1 A12 (k_big_trsm), and the trailing block takes the MFMA update
-// (k_big_gemm).  Per entry the arithmetic is that of right-looking
-// elimination: a_ik -= l_iq u_qk for q = k0 .. k-1 in order, then l_ik = a_ik / u_kk.
-constexpr int kBigPanel = 64;
+// Two-level blocked LU of fronts above 128 slots: outer panels of kBigOuter
+// pivots (groups whose largest front has at least kTwoLevelMin slots; else
+// outer = inner = kBigPanel, the plain right-looking scheme), each factored
+// as inner panels of kBigPanel.  Per inner panel: the
+// diagonal block A11 (one wavefront per front, multipliers stored below its
+// diagonal; k_big_diag), L21 = A21 U11^-1 for every row below
+// (k_big_trsm_rows), U12 = L11^-1 A12 for the columns inside the outer panel
+// only (k_big_trsm) and their MFMA update (k_big_gemm with a column limit).
+// Then the outer panel's rows are solved for the columns to its right block
+// by block (k_big_gemm_rows, k_big_trsm), and the trailing block takes one
+// MFMA update with K = kBigOuter (k_big_gemm), so the trailing matrix is read
+// and written once per outer panel instead of once per inner panel.
+constexpr int kBigPanel = 64, kBigOuter = 256, kTwoLevelMin = 4096;
 
 template <typename T>
 __device__ inline T pivot_recip(T piv, int* fail, int inst, bool flag) {
@@ -497,30 +503,50 @@ __device__ inline T pivot_recip(T piv, int* fail, int inst, bool flag) {
     return T(1) / piv;
 }
 
+// The diagonal block of one front per workgroup of one wavefront: lane t
+// holds column t of the block in registers; step k broadcasts column k
+// (lane k's registers) with readlane, so no barrier is needed.  Same formula
+// per entry as the LDS kernels: l_i = a_ik / a_kk, a_ij -= l_i u_kj.
 template <typename T>
-__global__ void __launch_bounds__(256) k_big_diag(FrontArgs a, int begin, int k0, T* front) {
-    __shared__ T D[kBigPanel][kBigPanel + 1];
+__device__ inline T lane_bcast(T v, int src) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+    } else {
+        long long const b = __builtin_bit_cast(long long, v);
+        int const lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffLL), src);
+        int const hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), src);
+        return __builtin_bit_cast(T, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kWave) k_big_diag(FrontArgs a, int begin, int k0, T* front) {
     FrontMeta const fm = a.meta[begin + blockIdx.x];
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
     if (k0 >= p) return;
     int const kb = min(kBigPanel, p - k0);
-    int const tid = threadIdx.x;
-    T* F = front + fm.off;
-    for (int e = tid; e < kb * kb; e += 256) D[e / kb][e % kb] = F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb];
-    __syncthreads();
-    for (int k = 0; k < kb; ++k) {
-        T const rp = pivot_recip(D[k][k], a.fail, fm.inst, tid == 0);
-        int const c = kb - k - 1;
-        if (tid < c) D[k + 1 + tid][k] *= rp;  // the multipliers l_ik
-        __syncthreads();
-        for (int e = tid; e < c * c; e += 256) {
-            int const i = k + 1 + e / c, j = k + 1 + e % c;
-            D[i][j] -= D[i][k] * D[k][j];
+    int const t = threadIdx.x;
+    T* F = front + fm.off + static_cast<long long>(k0) * m + k0;
+    T col[kBigPanel];
+#pragma unroll
+    for (int i = 0; i < kBigPanel; ++i) col[i] = (t < kb && i < kb) ? F[static_cast<long long>(i) * m + t] : T(0);
+#pragma unroll
+    for (int k = 0; k < kBigPanel; ++k) {
+        if (k < kb) {  // uniform
+            T const rp = pivot_recip(lane_bcast(col[k], k), a.fail, fm.inst, t == 0);
+            T const ukt = col[k];
+#pragma unroll
+            for (int i = k + 1; i < kBigPanel; ++i) {
+                T const l = lane_bcast(col[i], k) * rp;
+                if (t > k) col[i] -= l * ukt;
+                else if (t == k) col[i] = l;
+            }
         }
-        __syncthreads();
     }
-    for (int e = tid; e < kb * kb; e += 256) F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb] = D[e / kb][e % kb];
+#pragma unroll
+    for (int i = 0; i < kBigPanel; ++i)
+        if (t < kb && i < kb) F[static_cast<long long>(i) * m + t] = col[i];
 }
 
 // L21 = A21 U11^-1 for the rows below the panel: 64 rows per workgroup staged
@@ -560,21 +586,26 @@ __global__ void __launch_bounds__(256) k_big_trsm_rows(FrontArgs a, int begin, i
 
 // U12 = L11^-1 A12 for the columns right of the panel: a thread per column,
 // the column's panel segment in registers, L11 in LDS.
+// Columns [cb, ce): cb = kend (cbeg < 0) or min(cbeg, p); ce = m (cend < 0) or
+// min(cend, p) -- the two-level factorisation solves the columns inside the
+// outer panel first and the ones right of it later.
 template <typename T>
-__global__ void __launch_bounds__(256) k_big_trsm(FrontArgs a, int begin, int k0, int nb, T* front) {
+__global__ void __launch_bounds__(256) k_big_trsm(FrontArgs a, int begin, int k0, int nb, int cbeg, int cend,
+                                                  T* front) {
     __shared__ T Ls[kBigPanel][kBigPanel + 1];
     FrontMeta const fm = a.meta[begin + blockIdx.y];
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
     if (k0 >= p) return;
     int const kend = min(k0 + nb, p), kb = kend - k0;
-    if (kend + static_cast<int>(blockIdx.x) * 256 >= m) return;  // uniform: no columns for this block
+    int const cb = cbeg < 0 ? kend : min(cbeg, p), ce = cend < 0 ? m : min(cend, p);
+    if (cb + static_cast<int>(blockIdx.x) * 256 >= ce) return;  // uniform: no columns for this block
     T* F = front + fm.off;
     for (int e = threadIdx.x; e < kb * kb; e += 256)
         Ls[e / kb][e % kb] = F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb];
     __syncthreads();
-    int const j = kend + blockIdx.x * 256 + threadIdx.x;
-    if (j >= m) return;
+    int const j = cb + blockIdx.x * 256 + threadIdx.x;
+    if (j >= ce) return;
     T* cj = F + static_cast<long long>(k0) * m + j;
     T x[kBigPanel];
 #pragma unroll
@@ -613,27 +644,19 @@ struct Mfma<double> {
     __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
-// C[kend:m, kend:m] -= L21 * U12 (K = the panel): 128x128 tiles, four
-// wavefronts with 4x4 MFMA blocks each; A and B staged k-major in LDS in K
-// chunks (A read along its rows, so both loads are coalesced).
+// C[r0:r1, c0:c1] -= A[r0:r1, ka:ke] * B[ka:ke, c0:c1] inside one front: 128x128
+// tiles, four wavefronts with 4x4 MFMA blocks each; A and B staged k-major in
+// LDS in K chunks (A read along its rows, so both loads are coalesced).
 template <typename T>
-__global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0, int nb, T* front) {
+__device__ inline void gemm_tile(T* F, int m, int r0, int r1, int c0, int c1, int ka, int ke, int i0, int j0) {
     constexpr int TB = 128, KC = sizeof(T) == 4 ? 32 : 16;
     __shared__ T As[KC][TB + 4];
     __shared__ T Bs[KC][TB + 4];
-    FrontMeta const fm = a.meta[begin + blockIdx.z];
-    if (a.done && a.done[fm.inst]) return;
-    int const m = fm.m, p = fm.p;
-    if (k0 >= p) return;
-    int const kend = min(k0 + nb, p);
-    int const kb = kend - k0;
-    int const i0 = kend + blockIdx.y * TB, j0 = kend + blockIdx.x * TB;
-    if (i0 >= m || j0 >= m) return;
-    T* F = front + fm.off;
     int const tid = threadIdx.x;
     using V = typename Mfma<T>::V;
     int const wave = tid / kWave, lane = tid % kWave;
     int const wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+    int const kb = ke - ka;
     // the accumulators start from C and A enters negated, so the MFMAs leave
     // C - A B; the C loads overlap the K loop
     V acc[4][4];
@@ -644,20 +667,110 @@ __global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 int const gi = i0 + wr + u * 16 + Mfma<T>::row(lane, r), gj = j0 + wc + v * 16 + (lane & 15);
-                acc[u][v][r] = (gi < m && gj < m) ? F[static_cast<long long>(gi) * m + gj] : T(0);
+                acc[u][v][r] = (gi < r1 && gj < c1) ? F[static_cast<long long>(gi) * m + gj] : T(0);
             }
-    for (int c0 = 0; c0 < kb; c0 += KC) {
-        int const kc = min(KC, kb - c0);
+    for (int k1 = 0; k1 < kb; k1 += KC) {
+        int const kc = min(KC, kb - k1);
         __syncthreads();  // the previous chunk has been consumed
         for (int e = tid; e < KC * TB; e += 256) {
             int const ar = e / KC, ak = e % KC;  // A: along the row
             int const gi = i0 + ar;
-            As[ak][ar] = (ak < kc && gi < m) ? -F[static_cast<long long>(gi) * m + k0 + c0 + ak] : T(0);
+            As[ak][ar] = (ak < kc && gi < r1) ? -F[static_cast<long long>(gi) * m + ka + k1 + ak] : T(0);
             int const bx = e % TB, bk = e / TB;  // B: along the row of U12
             int const gj = j0 + bx;
-            Bs[bk][bx] = (bk < kc && gj < m) ? F[static_cast<long long>(k0 + c0 + bk) * m + gj] : T(0);
+            Bs[bk][bx] = (bk < kc && gj < c1) ? F[static_cast<long long>(ka + k1 + bk) * m + gj] : T(0);
         }
         __syncthreads();
+        for (int kk = 0; kk < kc; kk += 4) {
+            int const k = kk + (lane >> 4);
+            T av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) av[u] = As[k][wr + u * 16 + (lane & 15)];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) bv[v] = Bs[k][wc + v * 16 + (lane & 15)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[u][v] = Mfma<T>::mma(av[u], bv[v], acc[u][v]);
+        }
+    }
+    // store addresses recomputed from an opaque copy of the lane index, so the
+    // load addresses are not kept live across the K loop
+    int sl = lane;
+    asm volatile("" : "+v"(sl));
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int const gi = i0 + wr + u * 16 + Mfma<T>::row(sl, r), gj = j0 + wc + v * 16 + (sl & 15);
+                if (gi < r1 && gj < c1) F[static_cast<long long>(gi) * m + gj] = acc[u][v][r];
+            }
+}
+
+// Trailing update of a panel [k0, kend), kend = min(k0 + nb, p): rows [kend, m),
+// columns [kend, ce) with ce = m (cend < 0) or min(cend, p).  Its own copy of
+// the tile loop (a single bound per direction keeps it within the register
+// file without spills).
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0, int nb, int cend, T* front) {
+    constexpr int TB = 128, KC = sizeof(T) == 4 ? 32 : 16;
+    __shared__ T As[KC][TB + 4];
+    __shared__ T Bs[KC][TB + 4];
+    FrontMeta const fm = a.meta[begin + blockIdx.z];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
+    if (k0 >= p) return;
+    int const kend = min(k0 + nb, p), ce = cend < 0 ? m : min(cend, p);
+    int const kb = kend - k0;
+    int const i0 = kend + blockIdx.y * TB, j0 = kend + blockIdx.x * TB;
+    if (i0 >= m || j0 >= ce) return;
+    T* F = front + fm.off;
+    int const tid = threadIdx.x;
+    using V = typename Mfma<T>::V;
+    int const wave = tid / kWave, lane = tid % kWave;
+    int const wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
+    V acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int const gi = i0 + wr + u * 16 + Mfma<T>::row(lane, r), gj = j0 + wc + v * 16 + (lane & 15);
+                acc[u][v][r] = (gi < m && gj < ce) ? F[static_cast<long long>(gi) * m + gj] : T(0);
+            }
+    // K chunks double-buffered through registers: the next chunk's loads
+    // (clamped addresses, unconditional; the bounds are applied when the
+    // values enter LDS) are in flight while the current chunk's MFMAs run
+    constexpr int PER = KC * TB / 256;
+    T ra[PER], rb[PER];
+    auto load_chunk = [&](int c0) {
+        int const kc = min(KC, kb - c0);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int const e = tid + q * 256;
+            int const ar = e / KC, ak = e % KC;  // A: along the row
+            ra[q] = F[static_cast<long long>(min(i0 + ar, m - 1)) * m + k0 + c0 + min(ak, kc - 1)];
+            int const bx = e % TB, bk = e / TB;  // B: along the row of U12
+            rb[q] = F[static_cast<long long>(k0 + c0 + min(bk, kc - 1)) * m + min(j0 + bx, ce - 1)];
+        }
+    };
+    load_chunk(0);
+    for (int c0 = 0; c0 < kb; c0 += KC) {
+        int const kc = min(KC, kb - c0);
+        __syncthreads();  // the previous chunk has been consumed
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            int const e = tid + q * 256;
+            int const ar = e / KC, ak = e % KC;
+            As[ak][ar] = (ak < kc && i0 + ar < m) ? -ra[q] : T(0);
+            int const bx = e % TB, bk = e / TB;
+            Bs[bk][bx] = (bk < kc && j0 + bx < ce) ? rb[q] : T(0);
+        }
+        __syncthreads();
+        if (c0 + KC < kb) load_chunk(c0 + KC);
         for (int kk = 0; kk < kc; kk += 4) {
             int const k = kk + (lane >> 4);
             T av[4], bv[4];
@@ -678,8 +791,23 @@ __global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 int const gi = i0 + wr + u * 16 + Mfma<T>::row(lane, r), gj = j0 + wc + v * 16 + (lane & 15);
-                if (gi < m && gj < m) F[static_cast<long long>(gi) * m + gj] = acc[u][v][r];
+                if (gi < m && gj < ce) F[static_cast<long long>(gi) * m + gj] = acc[u][v][r];
             }
+}
+
+// Row block [k0, min(k0 + 64, p)) of an outer panel starting at j, columns
+// [jend, m) with jend = min(j + nbo, p): subtracts the earlier blocks'
+// contribution, L[k0.., j:k0] U[j:k0, jend..] (K = k0 - j).
+template <typename T>
+__global__ void __launch_bounds__(256) k_big_gemm_rows(FrontArgs a, int begin, int j, int nbo, int k0, T* front) {
+    FrontMeta const fm = a.meta[begin + blockIdx.z];
+    if (a.done && a.done[fm.inst]) return;
+    int const m = fm.m, p = fm.p;
+    if (k0 >= p) return;
+    int const jend = min(j + nbo, p), r1 = min(k0 + kBigPanel, p);
+    int const j0 = jend + blockIdx.x * 128;
+    if (j0 >= m) return;
+    gemm_tile<T>(front + fm.off, m, k0, r1, jend, m, j, k0, k0, j0);
 }
 
 // Forward solve L y = b (TR: U^T z = b), front by front from the leaves (large
