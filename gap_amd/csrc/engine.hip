@@ -732,7 +732,8 @@ struct FrontRunner {
     }
     static void big_backward(DevFronts const&, FrontArgs const& a, int beg, int nb, int pmax, T const* front, T* vec,
                              double* sol, hipStream_t s) {
-        hipLaunchKernelGGL(k_big_bwd_init<T>, dim3(blocks_for(pmax, 256), nb), dim3(256), 0, s, a, beg, front, vec, sol);
+        hipLaunchKernelGGL(k_big_bwd_init<T>, dim3(blocks_for(pmax, kBwdInitRows), nb), dim3(kBwdInitRows * kWave), 0,
+                           s, a, beg, front, vec, sol);
         for (int k0 = ((pmax - 1) / kSolvePanel) * kSolvePanel; k0 >= 0; k0 -= kSolvePanel) {
             hipLaunchKernelGGL(k_big_bwd_diag<T>, dim3(nb), dim3(kWave), 0, s, a, beg, k0, front, vec, sol);
             if (k0 > 0)

@@ -950,26 +950,35 @@ __global__ void __launch_bounds__(256) k_big_fwd_update(FrontArgs a, int begin, 
     int const i = kend + blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
     const T* ri = front + fm.off + static_cast<long long>(i) * m + k0;
+    T r[kSolvePanel];
+#pragma unroll
+    for (int k = 0; k < kSolvePanel; ++k) r[k] = ri[min(k, kb - 1)];  // issued together, in bounds
     T acc = w[i];
-    for (int k = 0; k < kb; ++k) acc -= ri[k] * y[k];
+#pragma unroll
+    for (int k = 0; k < kSolvePanel; ++k)
+        if (k < kb) acc -= r[k] * y[k];
     w[i] = acc;
 }
 
-// pivot rows: w_k -= U[k][t] x_t over the structure slots t = p .. m-1
+// pivot rows: w_k -= sum_t U[k][t] x_t over the structure slots t = p .. m-1;
+// one wavefront per row, lanes along the row (coalesced), summed by a fixed
+// butterfly so every solve adds in the same order
+constexpr int kBwdInitRows = 4;  // rows (wavefronts) per workgroup
 template <typename T>
-__global__ void __launch_bounds__(256) k_big_bwd_init(FrontArgs a, int begin, const T* front, T* vec,
-                                                      const double* sol) {
+__global__ void __launch_bounds__(kBwdInitRows * kWave) k_big_bwd_init(FrontArgs a, int begin, const T* front,
+                                                                       T* vec, const double* sol) {
     FrontMeta const fm = a.meta[begin + blockIdx.y];
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
-    int const k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= p || p == m) return;
+    int const lane = threadIdx.x % kWave;
+    int const k = blockIdx.x * kBwdInitRows + threadIdx.x / kWave;
+    if (k >= p || p == m) return;  // uniform per wavefront
     const int* cols = a.slot_col + fm.slot_beg;
     const T* rk = front + fm.off + static_cast<long long>(k) * m;
-    T* w = vec + fm.voff;
-    T acc = w[k];
-    for (int t = p; t < m; ++t) acc -= rk[t] * static_cast<T>(sol[cols[t]]);
-    w[k] = acc;
+    T acc = T(0);
+    for (int t = p + lane; t < m; t += kWave) acc += rk[t] * static_cast<T>(sol[cols[t]]);
+    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) vec[fm.voff + k] -= acc;
 }
 
 // panel [k0, kend) from its last row: x_k = w_k / U[k][k], written to the
@@ -1022,8 +1031,13 @@ __global__ void __launch_bounds__(256) k_big_bwd_update(FrontArgs a, int begin, 
     int const i = blockIdx.x * 256 + threadIdx.x;
     if (i >= k0) return;
     const T* ri = front + fm.off + static_cast<long long>(i) * m + k0;
+    T r[kSolvePanel];
+#pragma unroll
+    for (int k = 0; k < kSolvePanel; ++k) r[k] = ri[min(k, kb - 1)];  // issued together, in bounds
     T acc = w[i];
-    for (int k = kb - 1; k >= 0; --k) acc -= ri[k] * x[k];
+#pragma unroll
+    for (int k = kSolvePanel - 1; k >= 0; --k)
+        if (k < kb) acc -= r[k] * x[k];
     w[i] = acc;
 }
 
