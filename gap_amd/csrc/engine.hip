@@ -699,8 +699,8 @@ struct FrontRunner {
             for (int k0 = j; k0 < jlim; k0 += kPanel) {
                 int const below = mmax - k0, inner = jlim - k0;
                 hipLaunchKernelGGL(k_big_diag<T>, dim3(nb), dim3(kWave), 0, s, a, beg, k0, front);
-                hipLaunchKernelGGL(k_big_trsm_rows<T>, dim3(blocks_for(below, 64), nb), dim3(256), 0, s, a, beg, k0,
-                                   front);
+                hipLaunchKernelGGL(k_big_trsm_rows<T>, dim3(blocks_for(below, kTrsmRowsBlock), nb),
+                                   dim3(kTrsmRowsBlock), 0, s, a, beg, k0, front);
                 if (inner <= kPanel) continue;  // no front has columns left inside the outer panel
                 hipLaunchKernelGGL(k_big_trsm<T>, dim3(blocks_for(inner, 256), nb), dim3(256), 0, s, a, beg, k0,
                                    kPanel, -1, j + outer, front);

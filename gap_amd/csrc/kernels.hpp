@@ -549,39 +549,47 @@ __global__ void __launch_bounds__(kWave) k_big_diag(FrontArgs a, int begin, int 
         if (t < kb && i < kb) F[static_cast<long long>(i) * m + t] = col[i];
 }
 
-// L21 = A21 U11^-1 for the rows below the panel: 64 rows per workgroup staged
-// in LDS with coalesced loads, a thread per row for the substitution, U11 in LDS.
+// L21 = A21 U11^-1 for the rows below the panel: a thread per row with the
+// row's panel segment in registers (unconditional, clamped loads) and U11
+// transposed in LDS, so each step reads a contiguous, broadcast column.
+constexpr int kTrsmRowsBlock = 256;
 template <typename T>
-__global__ void __launch_bounds__(256) k_big_trsm_rows(FrontArgs a, int begin, int k0, T* front) {
-    constexpr int RB = 64;
-    __shared__ T U[kBigPanel][kBigPanel + 1];
-    __shared__ T R[RB][kBigPanel + 1];
+__global__ void __launch_bounds__(kTrsmRowsBlock) k_big_trsm_rows(FrontArgs a, int begin, int k0, T* front) {
+    __shared__ T Ut[kBigPanel][kBigPanel];  // Ut[k][q] = U[q][k]
     __shared__ T Rp[kBigPanel];
     FrontMeta const fm = a.meta[begin + blockIdx.y];
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
     if (k0 >= p) return;
     int const kb = min(kBigPanel, p - k0), kend = k0 + kb;
-    int const r0 = kend + static_cast<int>(blockIdx.x) * RB;
+    int const r0 = kend + static_cast<int>(blockIdx.x) * kTrsmRowsBlock;
     if (r0 >= m) return;  // uniform: no rows for this block
-    int const nr = min(RB, m - r0);
     int const tid = threadIdx.x;
     T* F = front + fm.off;
-    for (int e = tid; e < kb * kb; e += 256) U[e / kb][e % kb] = F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb];
-    for (int e = tid; e < nr * kb; e += 256) R[e / kb][e % kb] = F[static_cast<long long>(r0 + e / kb) * m + k0 + e % kb];
+    for (int e = tid; e < kb * kb; e += kTrsmRowsBlock)
+        Ut[e % kb][e / kb] = F[static_cast<long long>(k0 + e / kb) * m + k0 + e % kb];
+    if (tid < kb) Rp[tid] = pivot_recip(F[static_cast<long long>(k0 + tid) * m + k0 + tid], nullptr, 0, false);
+    int const r = r0 + tid;
+    const T* row = F + static_cast<long long>(min(r, m - 1)) * m + k0;
+    T x[kBigPanel];
+#pragma unroll
+    for (int c = 0; c < kBigPanel; ++c) x[c] = row[min(c, kb - 1)];
     __syncthreads();
-    if (tid < kb) Rp[tid] = pivot_recip(U[tid][tid], nullptr, 0, false);
-    __syncthreads();
-    if (tid < nr) {
-        T* x = R[tid];
-        for (int k = 0; k < kb; ++k) {
+#pragma unroll
+    for (int k = 0; k < kBigPanel; ++k) {
+        if (k < kb) {  // uniform
             T acc = x[k];
-            for (int q = 0; q < k; ++q) acc -= x[q] * U[q][k];
+#pragma unroll
+            for (int q = 0; q < k; ++q) acc -= x[q] * Ut[k][q];
             x[k] = acc * Rp[k];
         }
     }
-    __syncthreads();
-    for (int e = tid; e < nr * kb; e += 256) F[static_cast<long long>(r0 + e / kb) * m + k0 + e % kb] = R[e / kb][e % kb];
+    if (r < m) {
+        T* out = F + static_cast<long long>(r) * m + k0;
+#pragma unroll
+        for (int c = 0; c < kBigPanel; ++c)
+            if (c < kb) out[c] = x[c];
+    }
 }
 
 // U12 = L11^-1 A12 for the columns right of the panel: a thread per column,
@@ -609,7 +617,10 @@ __global__ void __launch_bounds__(256) k_big_trsm(FrontArgs a, int begin, int k0
     T* cj = F + static_cast<long long>(k0) * m + j;
     T x[kBigPanel];
 #pragma unroll
-    for (int r = 0; r < kBigPanel; ++r) x[r] = r < kb ? cj[static_cast<long long>(r) * m] : T(0);
+    for (int r = 0; r < kBigPanel; ++r) {
+        T const v = cj[static_cast<long long>(min(r, kb - 1)) * m];  // unconditional, inside the panel rows
+        x[r] = r < kb ? v : T(0);
+    }
 #pragma unroll
     for (int r = 1; r < kBigPanel; ++r) {
         if (r < kb) {
@@ -950,13 +961,8 @@ __global__ void __launch_bounds__(256) k_big_fwd_update(FrontArgs a, int begin, 
     int const i = kend + blockIdx.x * 256 + threadIdx.x;
     if (i >= m) return;
     const T* ri = front + fm.off + static_cast<long long>(i) * m + k0;
-    T r[kSolvePanel];
-#pragma unroll
-    for (int k = 0; k < kSolvePanel; ++k) r[k] = ri[min(k, kb - 1)];  // issued together, in bounds
     T acc = w[i];
-#pragma unroll
-    for (int k = 0; k < kSolvePanel; ++k)
-        if (k < kb) acc -= r[k] * y[k];
+    for (int k = 0; k < kb; ++k) acc -= ri[k] * y[k];
     w[i] = acc;
 }
 
