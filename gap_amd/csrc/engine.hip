@@ -205,6 +205,10 @@ inline int front_hist_bin(int m) {
 // Device copy of a (possibly batched) multifrontal plan.  Fronts are numbered
 // in (level, size class) order so a launch covers a contiguous id range.
 struct DevFronts {
+    // optional second stream and two events for the look-ahead of the
+    // two-level large-front factorisation (null: everything on one stream)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
     int n_fronts = 0, n_levels = 0;
     std::vector<int> level_ptr;  // host copy of the level schedule
     // fronts of level l and class c: ids [cls_ptr[l*K+c], cls_ptr[l*K+c+1])
@@ -694,6 +698,8 @@ struct FrontRunner {
         // updates and solves cost more than they save): outer width 64 is the
         // plain right-looking scheme.
         int const outer = mmax >= kTwoLevelMin ? kBigOuter : kPanel;
+        bool const look = outer == kBigOuter && d.side && d.ev_a && d.ev_b;
+        bool pending = false;  // a side-stream update awaits its join
         for (int j = 0; j < pmax; j += outer) {
             int const jlim = std::min(j + outer, pmax);
             for (int k0 = j; k0 < jlim; k0 += kPanel) {
@@ -705,9 +711,13 @@ struct FrontRunner {
                 hipLaunchKernelGGL(k_big_trsm<T>, dim3(blocks_for(inner, 256), nb), dim3(256), 0, s, a, beg, k0,
                                    kPanel, -1, j + outer, front);
                 hipLaunchKernelGGL(k_big_gemm<T>, dim3(blocks_for(inner, 128), blocks_for(below, 128), nb), dim3(256),
-                                   0, s, a, beg, k0, kPanel, j + outer, front);
+                                   0, s, a, beg, k0, kPanel, -1, j + outer, front);
             }
             int const right = mmax - j;
+            // the previous panel's update of the columns right of this panel
+            // ran on the side stream meanwhile; this panel's own rows need it
+            if (pending) HIPCHK(hipStreamWaitEvent(s, d.ev_b, 0));
+            pending = false;
             for (int k0 = j; k0 < jlim; k0 += kPanel) {
                 if (k0 > j)
                     hipLaunchKernelGGL(k_big_gemm_rows<T>, dim3(blocks_for(right, 128), 1, nb), dim3(256), 0, s, a,
@@ -716,8 +726,24 @@ struct FrontRunner {
                                    kPanel, j + outer, -1, front);
             }
             int const tiles = blocks_for(right, 128);
-            hipLaunchKernelGGL(k_big_gemm<T>, dim3(tiles, tiles, nb), dim3(256), 0, s, a, beg, j, outer, -1, front);
+            if (look && j + outer < pmax) {
+                // look-ahead: the next outer panel's columns first, then the
+                // columns right of it on the side stream while the next panel
+                // is factored (disjoint columns: that panel touches only its own)
+                hipLaunchKernelGGL(k_big_gemm<T>, dim3(blocks_for(outer, 128), tiles, nb), dim3(256), 0, s, a, beg, j,
+                                   outer, -1, j + 2 * outer, front);
+                HIPCHK(hipEventRecord(d.ev_a, s));
+                HIPCHK(hipStreamWaitEvent(d.side, d.ev_a, 0));
+                hipLaunchKernelGGL(k_big_gemm<T>, dim3(tiles, tiles, nb), dim3(256), 0, d.side, a, beg, j, outer,
+                                   j + 2 * outer, -1, front);
+                HIPCHK(hipEventRecord(d.ev_b, d.side));
+                pending = true;
+            } else {
+                hipLaunchKernelGGL(k_big_gemm<T>, dim3(tiles, tiles, nb), dim3(256), 0, s, a, beg, j, outer, -1, -1,
+                                   front);
+            }
         }
+        if (pending) HIPCHK(hipStreamWaitEvent(s, d.ev_b, 0));
     }
     // blocked forward / backward solves of the fronts [beg, beg+nb) of a large-front group
     static void big_forward(DevFronts const&, FrontArgs const& a, int beg, int nb, int mmax, int pmax, T const* front,
@@ -937,6 +963,8 @@ struct gaphip_plan {
     ~gaphip_plan() {
         if (pinned_active) (void)hipHostFree(pinned_active);
         if (ev) (void)hipEventDestroy(ev);
+        if (fr.ev_a) (void)hipEventDestroy(fr.ev_a);
+        if (fr.ev_b) (void)hipEventDestroy(fr.ev_b);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (stream2) (void)hipStreamDestroy(stream2);
@@ -1032,6 +1060,9 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     HIPCHK(hipStreamCreateWithFlags(&P.stream2, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&P.fr.ev_a, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&P.fr.ev_b, hipEventDisableTiming));
+    P.fr.side = P.stream2;
     hipStream_t s = P.stream;
 
     std::vector<int> bus_inst(P.n_bus), ang(P.n_bus), mag(P.n_bus), peq(P.n_bus), qeq(P.n_bus);

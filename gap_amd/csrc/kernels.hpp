@@ -721,11 +721,12 @@ __device__ inline void gemm_tile(T* F, int m, int r0, int r1, int c0, int c1, in
 }
 
 // Trailing update of a panel [k0, kend), kend = min(k0 + nb, p): rows [kend, m),
-// columns [kend, ce) with ce = m (cend < 0) or min(cend, p).  Its own copy of
-// the tile loop (a single bound per direction keeps it within the register
-// file without spills).
+// columns [cb, ce) with cb = kend (cbeg < 0) or min(cbeg, p) and ce = m (cend < 0)
+// or min(cend, p).  Its own copy of the tile loop (a single bound per
+// direction keeps it within the register file without spills).
 template <typename T>
-__global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0, int nb, int cend, T* front) {
+__global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0, int nb, int cbeg, int cend,
+                                                  T* front) {
     constexpr int TB = 128, KC = sizeof(T) == 4 ? 32 : 16;
     __shared__ T As[KC][TB + 4];
     __shared__ T Bs[KC][TB + 4];
@@ -733,9 +734,9 @@ __global__ void __launch_bounds__(256) k_big_gemm(FrontArgs a, int begin, int k0
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
     if (k0 >= p) return;
-    int const kend = min(k0 + nb, p), ce = cend < 0 ? m : min(cend, p);
+    int const kend = min(k0 + nb, p), cb = cbeg < 0 ? kend : min(cbeg, p), ce = cend < 0 ? m : min(cend, p);
     int const kb = kend - k0;
-    int const i0 = kend + blockIdx.y * TB, j0 = kend + blockIdx.x * TB;
+    int const i0 = kend + blockIdx.y * TB, j0 = cb + blockIdx.x * TB;
     if (i0 >= m || j0 >= ce) return;
     T* F = front + fm.off;
     int const tid = threadIdx.x;
