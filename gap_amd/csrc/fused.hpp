@@ -591,9 +591,13 @@ __device__ inline double block_max(double m, double* red) {
     return m;
 }
 
-// One workgroup per network; dynamic LDS = NW * pool elements of T.
+// One workgroup per network; dynamic LDS = NW * pool elements of T.  With
+// FP32 factors the kernel is held to 128 VGPRs (4 wavefronts per SIMD; 36
+// bytes of spills per lane): the memory waits dominate, so the extra resident
+// wavefronts pay (12.9 against 13.7 ms on the 4,096-grid batch).  The FP64
+// kernel would spill 200 bytes and keeps its natural 188 registers.
 template <typename T, int NW>
-__global__ void __launch_bounds__(NW* kWave) k_nr_grid(FusedArgs a, MisArgs ma, UpdArgs ua, FrontArgs fa, T* front,
+__global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1) k_nr_grid(FusedArgs a, MisArgs ma, UpdArgs ua, FrontArgs fa, T* front,
                                                         T* vec) {
     constexpr int NT = NW * kWave;
     extern __shared__ __align__(16) unsigned char grid_smem[];
