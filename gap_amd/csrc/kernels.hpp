@@ -504,23 +504,14 @@ __device__ inline T pivot_recip(T piv, int* fail, int inst, bool flag) {
 }
 
 // The diagonal block of one front per workgroup of one wavefront: lane t
-// holds column t of the block in registers; step k broadcasts column k
-// (lane k's registers) with readlane, so no barrier is needed.  Same formula
-// per entry as the LDS kernels: l_i = a_ik / a_kk, a_ij -= l_i u_kj.
-template <typename T>
-__device__ inline T lane_bcast(T v, int src) {
-    if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
-    } else {
-        long long const b = __builtin_bit_cast(long long, v);
-        int const lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffLL), src);
-        int const hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), src);
-        return __builtin_bit_cast(T, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
-    }
-}
-
+// holds row t of the block in registers.  At step k lane k's row is final; it
+// is written once to LDS (vector stores) and read back by every lane as a
+// broadcast, each lane takes its multiplier l_t = a_tk / a_kk from its own row
+// and updates it: a_tj -= l_t u_kj, the same formula per entry as the LDS
+// kernels, with wavefront-level synchronisation only.
 template <typename T>
 __global__ void __launch_bounds__(kWave) k_big_diag(FrontArgs a, int begin, int k0, T* front) {
+    __shared__ __align__(16) T prow[kBigPanel];
     FrontMeta const fm = a.meta[begin + blockIdx.x];
     if (a.done && a.done[fm.inst]) return;
     int const m = fm.m, p = fm.p;
@@ -528,25 +519,39 @@ __global__ void __launch_bounds__(kWave) k_big_diag(FrontArgs a, int begin, int 
     int const kb = min(kBigPanel, p - k0);
     int const t = threadIdx.x;
     T* F = front + fm.off + static_cast<long long>(k0) * m + k0;
-    T col[kBigPanel];
+    T row[kBigPanel];
+    const T* src = F + static_cast<long long>(min(t, kb - 1)) * m;
 #pragma unroll
-    for (int i = 0; i < kBigPanel; ++i) col[i] = (t < kb && i < kb) ? F[static_cast<long long>(i) * m + t] : T(0);
+    for (int j = 0; j < kBigPanel; ++j) row[j] = src[min(j, kb - 1)];
 #pragma unroll
     for (int k = 0; k < kBigPanel; ++k) {
         if (k < kb) {  // uniform
-            T const rp = pivot_recip(lane_bcast(col[k], k), a.fail, fm.inst, t == 0);
-            T const ukt = col[k];
+            if (t == k) {
 #pragma unroll
-            for (int i = k + 1; i < kBigPanel; ++i) {
-                T const l = lane_bcast(col[i], k) * rp;
-                if (t > k) col[i] -= l * ukt;
-                else if (t == k) col[i] = l;
+                for (int j = k; j < kBigPanel; ++j) prow[j] = row[j];
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            T const rp = pivot_recip(prow[k], a.fail, fm.inst, t == 0);
+            if (t > k) {
+                T const l = row[k] * rp;
+                row[k] = l;
+#pragma unroll
+                for (int j = k + 1; j < kBigPanel; ++j) row[j] -= l * prow[j];
+            }
+            // every lane has read prow before the next step overwrites it
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
+    if (t < kb) {
+        T* dst = F + static_cast<long long>(t) * m;
 #pragma unroll
-    for (int i = 0; i < kBigPanel; ++i)
-        if (t < kb && i < kb) F[static_cast<long long>(i) * m + t] = col[i];
+        for (int j = 0; j < kBigPanel; ++j)
+            if (j < kb) dst[j] = row[j];
+    }
 }
 
 // L21 = A21 U11^-1 for the rows below the panel: a thread per row with the
