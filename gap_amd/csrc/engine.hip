@@ -234,6 +234,10 @@ struct DevFronts {
     DBuf<int4> fz_pairs;
     DBuf<double2> fz_ypair;  // Y value of each pair, refreshed at every solve
     long long fz_npairs = 0;
+    // -F of each right-hand-side entry (factor type), written by every mismatch
+    // pass; host: the equation of each entry, read once by the plan
+    DBuf<double> fz_frhs;
+    std::vector<int> fz_rhs_eq;
     DBuf<int2> fz_rhs, fz_ext, fz_vx, fz_sol;
     long long fz_list_bytes = 0;
     std::vector<double> fz_fixed, fz_welems;  // per instance, per Newton iteration (see build_grid)
@@ -278,6 +282,7 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     int const G = static_cast<int>(plans.size());
     int const nf = d.n_fronts, K = kNumClasses;
     constexpr long long kIntMax = 0x7fffffffLL;
+    d.fz_rhs_eq.clear();
     d.fz_ok = nf > 0 && d.max_front <= kClassMax[kGridClasses - 1] && d.front_entries < kIntMax &&
               d.vec_entries < kIntMax && n_index < kIntMax && n_values < kIntMax;
     if (!d.fz_ok) return;
@@ -409,9 +414,10 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     for (size_t k = 0; k < tasks.size(); ++k) {
         GridTask const& tk = tasks[k];
         int const g = meta[tk.front_beg].inst;
-        d.fz_fixed[g] += sizeof(GridTask) * 2.0 + 64.0 * (tk.pair_end - tk.pair_beg) + 16.0 * (tk.rhs_end - tk.rhs_beg) +
+        d.fz_fixed[g] += sizeof(GridTask) * 2.0 + 64.0 * (tk.pair_end - tk.pair_beg) + 4.0 * (tk.rhs_end - tk.rhs_beg) +
                          8.0 * (tk.ext_end - tk.ext_beg) + 8.0 * (tk.vx_end - tk.vx_beg) + 16.0 * (tk.sol_end - tk.sol_beg);
-        d.fz_welems[g] += (tk.ext_end - tk.ext_beg) + (tk.vx_end - tk.vx_beg);
+        // -F of each right-hand-side entry: written by the mismatch pass, read by the task
+        d.fz_welems[g] += 2.0 * (tk.rhs_end - tk.rhs_beg) + (tk.ext_end - tk.ext_beg) + (tk.vx_end - tk.vx_beg);
         for (int n = tk.front_beg; n < tk.front_beg + (tk.cls_cnt & 0xff); ++n) {
             double const m = meta[n].m, pp = meta[n].p;
             d.fz_fixed[g] += 2.0 * sizeof(FrontMeta) + 8.0 * pp;  // metadata twice, solution writes
@@ -432,6 +438,8 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     check2(ext, d.front_entries, "extend-add entry");
     check2(vx, d.vec_entries, "vector entry");
     d.fz_pool = pool;
+    d.fz_rhs_eq.resize(rhs.size());
+    for (size_t q = 0; q < rhs.size(); ++q) d.fz_rhs_eq[q] = rhs[q].x;
     // largest networks first: they bound the launch
     std::vector<int> order(G);
     for (int g = 0; g < G; ++g) order[g] = g;
@@ -464,6 +472,8 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     d.fz_pairs.upload(pairs, s);
     d.fz_npairs = static_cast<long long>(pairs.size());
     d.fz_ypair.alloc(d.fz_npairs);
+    d.fz_frhs.alloc(rhs.size());  // doubles: room for either factor type
+    HIPCHK(hipMemsetAsync(d.fz_frhs.p, 0, sizeof(double) * rhs.size(), s));
     d.fz_rhs.upload(rhs, s);
     d.fz_ext.upload(ext, s);
     d.fz_vx.upload(vx, s);
@@ -928,6 +938,7 @@ struct gaphip_plan {
     gaphip::DBuf<int> br_f, br_t, br_inst;  // in-service branch endpoints (global bus) or -1
     gaphip::DBuf<double> flow_zb, flow_ib, flow_out;  // branch flows (per instance bases, results)
     gaphip::DBuf<int> bus_type_d;
+    gaphip::DBuf<int2> brhs;  // grid schedule: right-hand-side list positions of each bus's equations
     gaphip::DBuf<double> adj_lossb, adj_gp, adj_gq, adj_gr, adj_gx, adj_gvm, adj_gva;
     gaphip::DBuf<double2> v, vpol, s, sspec, y_pu, y_given;
     std::vector<int> y_given_mode;  // per instance: 0 assemble, 1 siemens given, 2 per-unit given
@@ -1210,6 +1221,26 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     pack_fronts(plans, ent_base, key_row, key_col, P.n_var, P.j_nnz, P.fr, s, &items);
     P.bus_off_d.upload(P.bus_off, s);
     HIPCHK(hipStreamSynchronize(s));
+    if (P.fr.fz_ok) {
+        // the grid mismatch pass writes each -F where the factor tasks read
+        // it: per bus the right-hand-side list positions of its equations
+        std::vector<int> eqpos(std::max<long long>(P.n_var, 1), -1);
+        auto const& req = P.fr.fz_rhs_eq;
+        for (size_t q = 0; q < req.size(); ++q) {
+            int const e = req[q];
+            if (e < 0 || e >= P.n_var || eqpos[e] >= 0) throw Error(GAPHIP_E_ARG, "right-hand-side entry out of range");
+            eqpos[e] = static_cast<int>(q);
+        }
+        std::vector<int2> brhs(std::max<long long>(P.n_bus, 1), make_int2(-1, -1));
+        for (long long i = 0; i < P.n_bus; ++i) {
+            brhs[i] = make_int2(peq[i] >= 0 ? eqpos[peq[i]] : -1, qeq[i] >= 0 ? eqpos[qeq[i]] : -1);
+            if ((peq[i] >= 0 && brhs[i].x < 0) || (qeq[i] >= 0 && brhs[i].y < 0))
+                throw Error(GAPHIP_E_ARG, "equation without a pivot in the grid schedule");
+        }
+        P.brhs.upload(brhs, s);
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<int>().swap(P.fr.fz_rhs_eq);
+    }
 
     // per-bus / per-instance state and work arrays
     P.v.alloc(P.n_bus);
@@ -1412,7 +1443,8 @@ void launch_grid(gaphip_plan& P, hipStream_t st, int first, int count, FusedArgs
                                                " bytes of LDS per workgroup, more than the device allows");
     }
     za.order_off = first;
-    hipLaunchKernelGGL((k_nr_grid<T, NW>), dim3(count), dim3(NW * kWave), shm, st, za, ma, ua, fra, front, vec);
+    hipLaunchKernelGGL((k_nr_grid<T, NW>), dim3(count), dim3(NW * kWave), shm, st, za, ma, ua, fra, front, vec,
+                       za.tasks, za.lvl_ptr);
 }
 
 template <typename T>
@@ -1448,10 +1480,10 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
         hipLaunchKernelGGL(k_gather_ypair, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
                            static_cast<int>(d.fz_npairs), d.fz_pairs.p, P.y_pu.p, d.fz_ypair.p);
         FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
-                     d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
+                     d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_frhs.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
                      P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, P.dx.p};
         MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
-                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p};
+                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p, P.brhs.p, d.fz_frhs.p};
         UpdArgs ua{static_cast<int>(N), P.bus_inst.p, P.done.p, P.lu_fail.p, P.ang.p, P.mag.p, P.dx.p, P.v.p};
         FrontArgs fra = d.args(nullptr, P.lu_fail.p);
         gaphip_tuning const& tu = P.tuning;

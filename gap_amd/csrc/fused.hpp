@@ -49,6 +49,7 @@ struct FusedArgs {
     const int4* gfront;   // per front: (storage offset, vector offset, m | p << 16, first slot)
     const int4* pairs;    // (i, j, Y slot, LDS position | entry mask << 16)
     const double2* ypair; // per-unit Y value of each pair, in pair order (gathered once per solve)
+    const void* frhs;     // per right-hand-side entry: -F (factor type), written by the mismatch pass
     const int2* rhs;      // (equation index, LDS position)
     const int2* ext;      // (front storage index, LDS position)
     const int2* vx;       // (vector index, LDS position)
@@ -135,7 +136,10 @@ static_assert(kListPad >= kPairChunks * kWave && kListPad >= kExtChunks * kWave,
 // and sin/cos of arbitrary arguments, whose library code would dominate the
 // register budget of the grid kernel.
 
-// S_i = V_i conj((Y V)_i) and the P / Q mismatch rows; returns max |F| of the bus
+// S_i = V_i conj((Y V)_i) and the P / Q mismatch rows; returns max |F| of the
+// bus.  Each equation's -F is also written, in the factor type, where the
+// factor tasks read it (their right-hand-side lists, contiguous per task).
+template <typename T>
 __device__ inline double grid_mismatch_bus(MisArgs const& a, int i) {
 #pragma clang fp contract(off)
     double cr = 0.0, cim = 0.0;
@@ -168,15 +172,19 @@ __device__ inline double grid_mismatch_bus(MisArgs const& a, int i) {
     a.s[i] = make_double2(sr, si);
     double2 sp = a.sspec[i];
     int pe = a.peq[i], qe = a.qeq[i];
+    int2 const rp = a.brhs[i];
+    T* fr = static_cast<T*>(a.frhs);
     double m = 0.0;
     if (pe >= 0) {
         double fp = sr - sp.x;
         a.f[pe] = fp;
+        fr[rp.x] = static_cast<T>(-1.0 * fp);
         m = fabs(fp);
     }
     if (qe >= 0) {
         double fq = si - sp.y;
         a.f[qe] = fq;
+        fr[rp.y] = static_cast<T>(-1.0 * fq);
         m = fmax(m, fabs(fq));
     }
     return m;
@@ -251,23 +259,32 @@ __device__ inline void grid_update_bus(UpdArgs const& a, int i, bool zero) {
 
 
 // The loads of a task that depend only on its descriptor (the lane's front
-// record and list entries, the pairs' Y values).  A wavefront issues them one
-// task ahead, so a task waits only for its gathers of V, S, F and child data.
+// record and list entries, the pairs' Y values, the -F values of its
+// right-hand side).  A wavefront issues them one task ahead, so a task waits
+// only for its gathers of V, S and child data.
+template <typename T>
 struct TaskPre {
     int4 gf;                 // front record of the lane's front
     int4 pr[kPairChunks];    // pair list entries
     double2 yv[kPairChunks]; // their per-unit Y
     int2 ex[kExtChunks];     // extend-add entries
-    int2 rh, vv;             // right-hand-side and child-vector entries
+    int rh;                  // right-hand-side LDS position
+    T rv;                    // and its -F
+    int2 vv;                 // child-vector entry
     int2 so;                 // (backward) solution entries
 };
 
+// LDS position of right-hand-side entry q (its equation index is not needed)
+__device__ inline int rhs_pos(FusedArgs const& a, int q) {
+    return __builtin_nontemporal_load(reinterpret_cast<const int*>(a.rhs + q) + 1);
+}
+
 // Unconditional: entries past the task's lists are the next task's or padding.
-template <bool BWD>
-__device__ inline TaskPre load_pre(FusedArgs const& a, GridTask const& tk) {
+template <typename T, bool BWD>
+__device__ inline TaskPre<T> load_pre(FusedArgs const& a, GridTask const& tk) {
     int const lane = lane_id(), cls = tk.cls_cnt >> 8;
     int const gi = lane >> (2 + cls);  // 4 << cls lanes per front
-    TaskPre r;
+    TaskPre<T> r;
     r.gf = ld_stream(a.gfront + tk.front_beg + gi);
     if constexpr (BWD) {
         r.so = ld_stream(a.sol + tk.sol_beg + lane);
@@ -279,7 +296,8 @@ __device__ inline TaskPre load_pre(FusedArgs const& a, GridTask const& tk) {
         }
 #pragma unroll
         for (int k = 0; k < kExtChunks; ++k) r.ex[k] = ld_stream(a.ext + tk.ext_beg + k * kWave + lane);
-        r.rh = ld_stream(a.rhs + tk.rhs_beg + lane);
+        r.rh = rhs_pos(a, tk.rhs_beg + lane);
+        r.rv = __builtin_nontemporal_load(static_cast<T const*>(a.frhs) + tk.rhs_beg + lane);
         r.vv = ld_stream(a.vx + tk.vx_beg + lane);
     }
     return r;
@@ -290,13 +308,13 @@ __device__ inline TaskPre load_pre(FusedArgs const& a, GridTask const& tk) {
 // eliminated right-hand side w of every front.
 template <typename T, int MM, int L>
 __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, GridTask const& tk,
-                                 TaskPre const& pre, int g, T* front, T* vec, T* pool, int ld) {
+                                 TaskPre<T> const& pre, int g, T* front, T* vec, T* pool, int ld) {
     constexpr int G = kWave / L;
     int const lane = lane_id(), gi = lane / L, t = lane % L;
     int const rows = ld - 1, wbase = G * rows * ld, region = wbase + G * MM;
     int4 const* pr = pre.pr;
     int2 const* ex = pre.ex;
-    int2 const rh = pre.rh, vv = pre.vv;
+    int2 const vv = pre.vv;
     double2 const* yv = pre.yv;
     // gathers, unconditional: every list entry (the next task's and padding
     // included) holds valid indices; which entries are this task's is applied
@@ -306,7 +324,7 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     for (int k = 0; k < kPairChunks; ++k) pok[k] = tk.pair_beg + k * kWave + lane < tk.pair_end;
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k) eok[k] = tk.ext_beg + k * kWave + lane < tk.ext_end && ex[k].y >= 0;
-    bool const rok = tk.rhs_beg + lane < tk.rhs_end && rh.y >= 0;
+    bool const rok = tk.rhs_beg + lane < tk.rhs_end && pre.rh >= 0;
     bool const vok = tk.vx_beg + lane < tk.vx_end && vv.y >= 0;
     // a lane whose value is not used reads lane 0's element instead, so its
     // load falls into a line the wavefront fetches anyway
@@ -323,7 +341,6 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k)
         exv[k] = ld_stream(front + (eok[k] ? ex[k].x : __builtin_amdgcn_readfirstlane(ex[k].x)));
-    double const fv = ma.f[rok ? rh.x : __builtin_amdgcn_readfirstlane(rh.x)];
     T const vxv = vec[vok ? vv.x : __builtin_amdgcn_readfirstlane(vv.x)];
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
@@ -346,11 +363,9 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
         int4 const p4 = ld_stream(a.pairs + q);
         put_pair(true, p4, ld_stream(a.ypair + q), ma.v[p4.x], ma.v[p4.y], ma.s[p4.x]);
     }
-    if (rok) pool[rh.y] = static_cast<T>(-1.0 * fv);
-    for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave) {
-        int2 const e = ld_stream(a.rhs + q);
-        pool[e.y] = static_cast<T>(-1.0 * ma.f[e.x]);
-    }
+    if (rok) pool[pre.rh] = pre.rv;
+    for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave)
+        pool[rhs_pos(a, q)] = static_cast<T const*>(a.frhs)[q];
     wave_sync();
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k)
@@ -438,7 +453,7 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
 
 // Backward substitution of the pivot block of up to kWave / L fronts.
 template <typename T, int MM, int L>
-__device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs const& a, FrontArgs const& fa, GridTask const& tk, TaskPre const& pre,
+__device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs const& a, FrontArgs const& fa, GridTask const& tk, TaskPre<T> const& pre,
                                    T const* front, T const* vec, double* sol, T* pool, int ld) {
     constexpr int G = kWave / L, CH = MM < 16 ? MM : 16;
     int const lane = lane_id(), gi = lane / L, t = lane % L;
@@ -521,7 +536,7 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
 
 template <typename T, bool BWD>
 __device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int g, GridTask const& tk,
-                                 TaskPre const& pre, T* front, T* vec, T* pool) {
+                                 TaskPre<T> const& pre, T* front, T* vec, T* pool) {
     switch (tk.cls_cnt >> 8) {
         case 0:
             if constexpr (BWD) grid_backward_task<T, 4, 4>(a, fa, tk, pre, front, vec, a.dx, pool, 5);
@@ -557,19 +572,20 @@ __global__ void __launch_bounds__(256) k_gather_ypair(int n, const int4* pairs, 
 // value is copied while its load is outstanding).  The descriptors read past
 // qe are the next level's or padding; only their static lists are loaded.
 template <typename T, bool BWD, int NW>
-__device__ inline void run_level(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int g, int q, int qe,
+__device__ inline void run_level(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma,
+                                 const GridTask* __restrict__ tasks, int g, int q, int qe,
                                  T* front, T* vec, T* pool) {
     if (q >= qe) return;
-    GridTask t0 = a.tasks[q], t1 = a.tasks[q + NW];
-    TaskPre p0 = load_pre<BWD>(a, t0);
+    GridTask t0 = tasks[q], t1 = tasks[q + NW];
+    TaskPre<T> p0 = load_pre<T, BWD>(a, t0);
     for (;;) {
-        TaskPre const p1 = load_pre<BWD>(a, t1);
-        GridTask const t2 = a.tasks[q + 2 * NW];
+        TaskPre<T> const p1 = load_pre<T, BWD>(a, t1);
+        GridTask const t2 = tasks[q + 2 * NW];
         grid_task<T, BWD>(a, fa, ma, g, t0, p0, front, vec, pool);
         q += NW;
         if (q >= qe) break;
-        TaskPre const p2 = load_pre<BWD>(a, t2);
-        GridTask const t3 = a.tasks[q + 2 * NW];
+        TaskPre<T> const p2 = load_pre<T, BWD>(a, t2);
+        GridTask const t3 = tasks[q + 2 * NW];
         grid_task<T, BWD>(a, fa, ma, g, t1, p1, front, vec, pool);
         q += NW;
         if (q >= qe) break;
@@ -592,13 +608,19 @@ __device__ inline double block_max(double m, double* red) {
 }
 
 // One workgroup per network; dynamic LDS = NW * pool elements of T.  With
-// FP32 factors the kernel is held to 128 VGPRs (4 wavefronts per SIMD; 36
-// bytes of spills per lane): the memory waits dominate, so the extra resident
-// wavefronts pay (12.9 against 13.7 ms on the 4,096-grid batch).  The FP64
-// kernel would spill 200 bytes and keeps its natural 188 registers.
+// FP32 factors the kernel is held to 128 VGPRs (4 wavefronts per SIMD): the
+// memory waits dominate, so the extra resident wavefronts pay.  The FP64
+// kernel keeps its natural register count (2 wavefronts per SIMD).
+//
+// The task descriptors and level ranges (a.tasks, a.lvl_ptr) come again as
+// restrict arguments: the kernel writes no memory they could alias, so the
+// compiler reads them with scalar loads, which are waited for on their own
+// counter.  Read as vector loads of a uniform address, each was followed by a
+// wait for every vector load in flight, the next task's prefetches included.
 template <typename T, int NW>
-__global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1) k_nr_grid(FusedArgs a, MisArgs ma, UpdArgs ua, FrontArgs fa, T* front,
-                                                        T* vec) {
+__global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1)
+    k_nr_grid(FusedArgs a, MisArgs ma, UpdArgs ua, FrontArgs fa, T* front, T* vec, const GridTask* __restrict__ tasks,
+              const int* __restrict__ lvl_ptr) {
     constexpr int NT = NW * kWave;
     extern __shared__ __align__(16) unsigned char grid_smem[];
     __shared__ double red[NW];
@@ -609,7 +631,7 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1) k_nr_grid(F
     T* pool = reinterpret_cast<T*>(grid_smem) + wave * a.pool;
     for (int it = 0; it < a.max_it; ++it) {
         double m = 0.0;
-        for (int i = b0 + tid; i < b1; i += NT) m = fmax(m, grid_mismatch_bus(ma, i));
+        for (int i = b0 + tid; i < b1; i += NT) m = fmax(m, grid_mismatch_bus<T>(ma, i));
         m = block_max<NW>(m, red);  // its barriers also publish S and F
         if (tid == 0) {
             a.maxf[static_cast<long long>(it) * a.n_inst + g] = dbits(m);
@@ -624,11 +646,11 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1) k_nr_grid(F
             break;
         }
         for (int e = e0; e < e1; ++e) {
-            run_level<T, false, NW>(a, fa, ma, g, a.lvl_ptr[e] + wave, a.lvl_ptr[e + 1], front, vec, pool);
+            run_level<T, false, NW>(a, fa, ma, tasks, g, lvl_ptr[e] + wave, lvl_ptr[e + 1], front, vec, pool);
             __syncthreads();
         }
         for (int e = e1 - 1; e >= e0; --e) {
-            run_level<T, true, NW>(a, fa, ma, g, a.lvl_ptr[e] + wave, a.lvl_ptr[e + 1], front, vec, pool);
+            run_level<T, true, NW>(a, fa, ma, tasks, g, lvl_ptr[e] + wave, lvl_ptr[e + 1], front, vec, pool);
             __syncthreads();
         }
         bool const zero = a.lu_fail[g] != 0;

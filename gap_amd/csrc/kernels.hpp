@@ -127,6 +127,10 @@ struct MisArgs {
     double2* vpol;  // (|V|, arg V) for the Jacobian
     double* f;
     unsigned long long* maxf;  // this iteration's row of per-instance maxima
+    // grid schedule only: the mismatch pass also writes each -F, in the
+    // factor type, in the order the factor tasks read it
+    const int2* brhs;  // per bus: right-hand-side list positions of its P and Q equations (-1: none)
+    void* frhs;        // one entry of the factor type per right-hand-side list entry
 };
 
 __device__ inline unsigned long long dbits(double x) { return __double_as_longlong(x); }
