@@ -158,9 +158,10 @@ int gaphip_plan_set_schedule(gaphip_plan* plan, int schedule, int* effective);
 /* Launch shape of the grid schedule.  A network whose elimination tree holds at
  * least grid_big_work front slots plus factor entries gets grid_big_waves
  * wavefronts per workgroup, the others grid_waves (each 1, 2, 4 or 8); the two
- * groups run concurrently.  grid_big_work < 0 puts every network in the first
- * group.  Results do not depend on the tuning.  Defaults: 2, 4, 20000 for
- * GAPHIP_MIXED plans, 2, 8, -1 for GAPHIP_FP64 plans.
+ * groups run concurrently, the second launched a few microseconds after the
+ * first.  grid_big_work < 0 puts every network in the first group.  Results
+ * do not depend on the tuning.  Defaults: 2, 4, 40000 for GAPHIP_MIXED plans,
+ * 2, 8, -1 for GAPHIP_FP64 plans.
  * gap_amd.tuning.autotune() times candidates on a plan and sets the fastest.
  * No counterpart in the reference. */
 typedef struct gaphip_tuning {
