@@ -136,56 +136,78 @@ static_assert(kListPad >= kPairChunks * kWave && kListPad >= kExtChunks * kWave,
 // and sin/cos of arbitrary arguments, whose library code would dominate the
 // register budget of the grid kernel.
 
-// S_i = V_i conj((Y V)_i) and the P / Q mismatch rows; returns max |F| of the
-// bus.  Each equation's -F is also written, in the factor type, where the
-// factor tasks read it (their right-hand-side lists, contiguous per task).
-template <typename T>
-__device__ inline double grid_mismatch_bus(MisArgs const& a, int i) {
+// S_i = V_i conj((Y V)_i) and the P / Q mismatch rows of buses i[0 .. B-1]
+// (those past the first only where use_rest); returns their max |F|.  Each
+// equation's -F is also written, in the factor type, where the factor tasks
+// read it (their right-hand-side lists, contiguous per task).  The B rows'
+// loads are issued together, four entries per row and step; the sums keep the
+// row order.  B = 2 pays with FP64 factors; with FP32 factors its registers
+// would spill (the kernel is held to 128).
+template <typename T, int B>
+__device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B], bool use_rest) {
 #pragma clang fp contract(off)
-    double cr = 0.0, cim = 0.0;
-    int const q1 = a.rp[i + 1];
-    // four row entries per step: their loads are issued together, the sums
-    // keep the row order
-    for (int q = a.rp[i]; q < q1; q += kRowPad) {
-        // unconditional loads (the Y-bus arrays extend kRowPad entries past
-        // the last row, and every column index is a bus); entries past the
-        // row are not summed
-        int c[kRowPad];
-        double2 yv[kRowPad], vv[kRowPad];
+    double cr[B], cim[B];
+    int qb[B], qe[B];
+    int n = 0;
 #pragma unroll
-        for (int u = 0; u < kRowPad; ++u) {
-            c[u] = a.ci[q + u];
-            yv[u] = a.y[q + u];
-        }
-#pragma unroll
-        for (int u = 0; u < kRowPad; ++u) vv[u] = a.v[c[u]];
-#pragma unroll
-        for (int u = 0; u < kRowPad; ++u)
-            if (q + u < q1) {
-                cr += yv[u].x * vv[u].x - yv[u].y * vv[u].y;
-                cim += yv[u].x * vv[u].y + yv[u].y * vv[u].x;
-            }
+    for (int b = 0; b < B; ++b) {
+        cr[b] = cim[b] = 0.0;
+        qb[b] = a.rp[i[b]];
+        qe[b] = a.rp[i[b] + 1];
+        n = max(n, qe[b] - qb[b]);
     }
-    double2 vi = a.v[i];
-    double sr = vi.x * cr + vi.y * cim;
-    double si = vi.y * cr - vi.x * cim;
-    a.s[i] = make_double2(sr, si);
-    double2 sp = a.sspec[i];
-    int pe = a.peq[i], qe = a.qeq[i];
-    int2 const rp = a.brhs[i];
+    for (int s = 0; s < n; s += kRowPad) {
+        // unconditional loads: an index past its row is clamped into the
+        // kRowPad entries that follow it (the Y-bus arrays extend kRowPad
+        // entries past the last row, and every column index is a bus);
+        // entries past a row are not summed
+        int c[B][kRowPad];
+        double2 yv[B][kRowPad], vv[B][kRowPad];
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int u = 0; u < kRowPad; ++u) {
+                int const q = min(qb[b] + s + u, qe[b] + kRowPad - 1);
+                c[b][u] = a.ci[q];
+                yv[b][u] = a.y[q];
+            }
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int u = 0; u < kRowPad; ++u) vv[b][u] = a.v[c[b][u]];
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int u = 0; u < kRowPad; ++u)
+                if (qb[b] + s + u < qe[b]) {
+                    cr[b] += yv[b][u].x * vv[b][u].x - yv[b][u].y * vv[b][u].y;
+                    cim[b] += yv[b][u].x * vv[b][u].y + yv[b][u].y * vv[b][u].x;
+                }
+    }
     T* fr = static_cast<T*>(a.frhs);
     double m = 0.0;
-    if (pe >= 0) {
-        double fp = sr - sp.x;
-        a.f[pe] = fp;
-        fr[rp.x] = static_cast<T>(-1.0 * fp);
-        m = fabs(fp);
-    }
-    if (qe >= 0) {
-        double fq = si - sp.y;
-        a.f[qe] = fq;
-        fr[rp.y] = static_cast<T>(-1.0 * fq);
-        m = fmax(m, fabs(fq));
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        double2 const vi = a.v[i[b]];
+        double2 const sp = a.sspec[i[b]];
+        int const pe = a.peq[i[b]], qeq = a.qeq[i[b]];
+        int2 const rp = a.brhs[i[b]];
+        if (b > 0 && !use_rest) break;
+        double const sr = vi.x * cr[b] + vi.y * cim[b];
+        double const si = vi.y * cr[b] - vi.x * cim[b];
+        a.s[i[b]] = make_double2(sr, si);
+        if (pe >= 0) {
+            double const fp = sr - sp.x;
+            a.f[pe] = fp;
+            fr[rp.x] = static_cast<T>(-1.0 * fp);
+            m = fmax(m, fabs(fp));
+        }
+        if (qeq >= 0) {
+            double const fq = si - sp.y;
+            a.f[qeq] = fq;
+            fr[rp.y] = static_cast<T>(-1.0 * fq);
+            m = fmax(m, fabs(fq));
+        }
     }
     return m;
 }
@@ -631,7 +653,13 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1)
     T* pool = reinterpret_cast<T*>(grid_smem) + wave * a.pool;
     for (int it = 0; it < a.max_it; ++it) {
         double m = 0.0;
-        for (int i = b0 + tid; i < b1; i += NT) m = fmax(m, grid_mismatch_bus<T>(ma, i));
+        constexpr int MB = sizeof(T) == 8 ? 2 : 1;  // buses per thread and step
+        for (int i = b0 + tid; i < b1; i += MB * NT) {
+            int ib[MB];
+#pragma unroll
+            for (int b = 0; b < MB; ++b) ib[b] = min(i + b * NT, b1 - 1);
+            m = fmax(m, grid_mismatch_buses<T, MB>(ma, ib, i + (MB - 1) * NT < b1));
+        }
         m = block_max<NW>(m, red);  // its barriers also publish S and F
         if (tid == 0) {
             a.maxf[static_cast<long long>(it) * a.n_inst + g] = dbits(m);
