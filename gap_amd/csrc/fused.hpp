@@ -325,6 +325,23 @@ __device__ inline TaskPre<T> load_pre(FusedArgs const& a, GridTask const& tk) {
     return r;
 }
 
+// Division by a pivot in the grid kernel's factorisation and solves: with
+// FP32 factors the hardware reciprocal (1 ulp, one instruction against a
+// dozen and two mode switches for the correctly rounded quotient), whose
+// rounding the FP64 residual of the next Newton iteration absorbs like any
+// other FP32 rounding of the factors; with FP64 factors the correctly
+// rounded quotient.
+template <typename T>
+__device__ inline T pivot_recip(T d) {
+    if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(d);
+    else return T(1) / d;
+}
+template <typename T>
+__device__ inline T pivot_div(T x, T d) {
+    if constexpr (sizeof(T) == 4) return x * __builtin_amdgcn_rcpf(d);
+    else return x / d;
+}
+
 // Assemble, factor and forward-eliminate up to kWave / L fronts of one class.
 // Stores the U rows (diagonal onwards), the Schur complement and the
 // eliminated right-hand side w of every front.
@@ -426,7 +443,7 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
                 if (act && t == 0) fa.fail[g] = 1;
                 piv = T(1);
             }
-            T const rp = T(1) / piv;
+            T const rp = pivot_recip(piv);
             if (act && t > k && t < m) {
                 T const ukj = col[k];
                 T lt = T(0);
@@ -454,7 +471,7 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
             if (k < p && t == 0) fa.fail[g] = 1;
             piv = T(1);
         }
-        T const rp = T(1) / piv;
+        T const rp = pivot_recip(piv);
         if (k < p && t > k && t < m) {
             T const ukj = F[k * ld + t];
             for (int i = k + 1; i < m; ++i) F[i * ld + t] -= (F[i * ld + k] * rp) * ukj;
@@ -526,7 +543,7 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
 #pragma unroll
         for (int k = MM - 1; k >= 0; --k) {
             if (k < p_max) {
-                T const xk = __shfl(acc / urow[k], gi * L + k);
+                T const xk = __shfl(pivot_div(acc, urow[k]), gi * L + k);
                 if (k < p) {
                     if (t < k) acc -= urow[k] * xk;
                     if (t == k) xme = xk;
@@ -547,7 +564,7 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
     int const p_max = wave_group_max<L>(p);
     for (int k = p_max - 1; k >= 0; --k) {
         if (k < p) {
-            T const xk = w[k] / F[k * ld + k];
+            T const xk = pivot_div(w[k], F[k * ld + k]);
             if (t < k) w[t] -= F[t * ld + k] * xk;
             if (t == k) xme = xk;
         }
