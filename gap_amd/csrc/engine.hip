@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -329,24 +330,51 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     // tasks per (instance, level), classes in increasing order
     std::vector<int> lvl_beg(G + 1, 0);
     for (int g = 0; g < G; ++g) lvl_beg[g + 1] = lvl_beg[g] + plans[g]->n_levels;
-    std::vector<std::vector<int2>> per(lvl_beg[G]);
+    // the fronts of each (instance, level), by size class
+    std::vector<std::array<std::vector<int>, kGridClasses>> byc(lvl_beg[G]);
     int m4 = 0;
     for (int l = 0; l < d.n_levels; ++l)
         for (int c = 0; c < kGridClasses; ++c) {
             size_t const b = static_cast<size_t>(l) * K + c;
-            int const per_task = kWave / kClassMax[c];
-            for (int n = d.cls_ptr[b]; n < d.cls_ptr[b + 1];) {
+            for (int n = d.cls_ptr[b]; n < d.cls_ptr[b + 1]; ++n) {
                 int const g = meta[n].inst;
-                int e = n;
-                while (e < d.cls_ptr[b + 1] && meta[e].inst == g && e - n < per_task) ++e;
                 if (l >= plans[g]->n_levels) throw Error(GAPHIP_E_ARG, "front level outside its network's tree");
-                per[lvl_beg[g] + l].push_back(make_int2(n, (c << 8) | (e - n)));
-                if (c == kGridClasses - 1)
-                    for (int q = n; q < e; ++q) m4 = std::max(m4, meta[q].m);
-                n = e;
+                byc[lvl_beg[g] + l][c].push_back(n);
+                if (c == kGridClasses - 1) m4 = std::max(m4, meta[n].m);
             }
         }
     d.fz_ld4 = m4 + 1;
+    // Tasks of each (instance, level): classes from the largest down, and the
+    // free lane groups of a class's last task take fronts of the smaller
+    // classes (a front fits a larger class's group).  A task costs about the
+    // same whatever its fill (its time is memory round trips), so fewer tasks
+    // per level means fewer rounds for the level's wavefronts.  tfront lists
+    // the fronts in task order; a task is a range of it.
+    std::vector<int> tfront;
+    tfront.reserve(nf);
+    std::vector<std::vector<int2>> per(lvl_beg[G]);
+    for (size_t e = 0; e < per.size(); ++e) {
+        auto& bc = byc[e];
+        for (int c = kGridClasses - 1; c >= 0; --c) {
+            int const per_task = kWave / kClassMax[c];
+            auto const& F = bc[c];
+            size_t i = 0;
+            while (i < F.size()) {
+                int const beg = static_cast<int>(tfront.size());
+                int cnt = 0;
+                for (; i < F.size() && cnt < per_task; ++i, ++cnt) tfront.push_back(F[i]);
+                for (int c2 = c - 1; c2 >= 0 && cnt < per_task; --c2) {
+                    auto& F2 = bc[c2];
+                    for (; !F2.empty() && cnt < per_task; ++cnt) {
+                        tfront.push_back(F2.back());
+                        F2.pop_back();
+                    }
+                }
+                per[e].push_back(make_int2(beg, (c << 8) | cnt));
+            }
+        }
+    }
+    if (static_cast<int>(tfront.size()) != nf) throw Error(GAPHIP_E_ARG, "grid tasks do not cover every front once");
     std::vector<int> lvl_ptr(1, 0);
     std::vector<GridTask> tasks;
     std::vector<int4> pairs;
@@ -367,9 +395,9 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
             tk.vx_beg = static_cast<int>(vx.size());
             tk.sol_beg = static_cast<int>(sol.size());
             for (int gi = 0; gi < cnt; ++gi) {
-                int const n = tr.x + gi;
+                int const n = tfront[tr.x + gi];
                 FrontMeta const& fm = meta[n];
-                if (front_class(fm.m) != c || fm.m > rows) throw Error(GAPHIP_E_ARG, "grid task class mismatch");
+                if (front_class(fm.m) > c || fm.m > rows) throw Error(GAPHIP_E_ARG, "grid task class mismatch");
                 int const tile = gi * rows * ld, wpos = wbase + gi * L;
                 auto in_tile = [&](int r, int col) { return r >= 0 && col >= 0 && r < fm.m && col < fm.m; };
                 for (int q = fpair_ptr[n]; q < fpair_ptr[n + 1]; ++q) {
@@ -413,12 +441,13 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     d.fz_welems.assign(G, 0.0);
     for (size_t k = 0; k < tasks.size(); ++k) {
         GridTask const& tk = tasks[k];
-        int const g = meta[tk.front_beg].inst;
+        int const g = meta[tfront[tk.front_beg]].inst;
         d.fz_fixed[g] += sizeof(GridTask) * 2.0 + 64.0 * (tk.pair_end - tk.pair_beg) + 4.0 * (tk.rhs_end - tk.rhs_beg) +
                          8.0 * (tk.ext_end - tk.ext_beg) + 8.0 * (tk.vx_end - tk.vx_beg) + 16.0 * (tk.sol_end - tk.sol_beg);
         // -F of each right-hand-side entry: written by the mismatch pass, read by the task
         d.fz_welems[g] += 2.0 * (tk.rhs_end - tk.rhs_beg) + (tk.ext_end - tk.ext_beg) + (tk.vx_end - tk.vx_beg);
-        for (int n = tk.front_beg; n < tk.front_beg + (tk.cls_cnt & 0xff); ++n) {
+        for (int q = tk.front_beg; q < tk.front_beg + (tk.cls_cnt & 0xff); ++q) {
+            int const n = tfront[q];
             double const m = meta[n].m, pp = meta[n].p;
             d.fz_fixed[g] += 2.0 * sizeof(FrontMeta) + 8.0 * pp;  // metadata twice, solution writes
             // stored U rows and Schur complement, w; then U rows and w read back
@@ -448,14 +477,16 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     });
     d.fz_work.resize(G);
     for (int k = 0; k < G; ++k) d.fz_work[k] = d.inst_factor[order[k]] + d.inst_slots[order[k]];
-    // compact per-front records of the grid kernel (offsets fit in int: fz_ok)
+    // compact per-front records of the grid kernel, in task order (offsets fit
+    // in int: fz_ok)
     std::vector<int4> gfront(nf);
-    for (int n = 0; n < nf; ++n) {
+    for (int k = 0; k < nf; ++k) {
+        int const n = tfront[k];
         FrontMeta const& fm = meta[n];
         if (fm.off < 0 || fm.off > kIntMax || fm.voff < 0 || fm.voff > kIntMax || fm.m < 0 || fm.m > 0x7fff ||
             fm.p < 0 || fm.p > fm.m)
             throw Error(GAPHIP_E_ARG, "front record out of range");
-        gfront[n] = make_int4(static_cast<int>(fm.off), static_cast<int>(fm.voff), fm.m | (fm.p << 16), fm.slot_beg);
+        gfront[k] = make_int4(static_cast<int>(fm.off), static_cast<int>(fm.voff), fm.m | (fm.p << 16), fm.slot_beg);
     }
     // padding read by the grid kernel's unconditional loads (never executed)
     tasks.insert(tasks.end(), kTaskPad, GridTask{});
