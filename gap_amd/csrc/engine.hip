@@ -466,7 +466,7 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     check2(sol, n_index, "solution entry");
     check2(ext, d.front_entries, "extend-add entry");
     check2(vx, d.vec_entries, "vector entry");
-    d.fz_pool = pool;
+    d.fz_pool = (pool + 3) / 4 * 4;  // whole 16-byte units per wavefront (the task clears its region with 16-byte stores)
     d.fz_rhs_eq.resize(rhs.size());
     for (size_t q = 0; q < rhs.size(); ++q) d.fz_rhs_eq[q] = rhs[q].x;
     // largest networks first: they bound the launch

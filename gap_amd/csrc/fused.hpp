@@ -384,7 +384,13 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
     wave_sync();  // the previous task of this wavefront has finished with the region
-    for (int e = lane; e < region; e += kWave) pool[e] = T(0);
+    {
+        // 16-byte stores: the pool of a wavefront starts 16-byte aligned and
+        // holds the region rounded up to whole 16-byte units
+        constexpr int V = 16 / static_cast<int>(sizeof(T));
+        float4* z = reinterpret_cast<float4*>(pool);
+        for (int e = lane; e < (region + V - 1) / V; e += kWave) z[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     wave_sync();
     auto put_pair = [&](bool ok, int4 const& p4, double2 y2, double2 pi2, double2 pj2, double2 s2) {
         int const base = p4.w & 0xffff, mask = p4.w >> 16;
