@@ -430,11 +430,13 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     int const p_max = wave_group_max<L>(p);
     if constexpr (MM <= 16) {
         // dense elimination in registers: lane t holds column t (rows 0..MM-1)
-        // and w[t]; step k broadcasts column k and w[k] from the group's lane k
+        // and w[t]; step k broadcasts column k and w[k] from the group's lane k.
+        // The tile is cleared past the front, so rows and columns past m are
+        // zero and stay zero under the updates (no bounds masks needed).
         T col[MM];
 #pragma unroll
-        for (int r = 0; r < MM; ++r) col[r] = (t < m && r < m) ? F[r * ld + t] : T(0);
-        T wt = t < m ? w[t] : T(0);
+        for (int r = 0; r < MM; ++r) col[r] = F[r * ld + t];
+        T wt = w[t];
 #pragma unroll
         for (int k = 0; k < MM; ++k) {
             if (k >= p_max) continue;  // uniform; keeps the loop fully unrolled
@@ -455,7 +457,7 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
                 T lt = T(0);
 #pragma unroll
                 for (int i = k + 1; i < MM; ++i) {
-                    if (i < m) col[i] -= (lk[i] * rp) * ukj;
+                    col[i] -= (lk[i] * rp) * ukj;
                     if (i == t) lt = lk[i];
                 }
                 wt -= (lt * rp) * wk;
