@@ -104,6 +104,7 @@ def main():
 
     # the same plan on the level-synchronous schedule (one launch per phase and tree level)
     levels_value = None
+    level_kernels = {}
     if plan.schedule == "grid":
         plan.set_schedule("levels")
         plan.solve(tol, max_it, base, fetch_v=False)
@@ -113,8 +114,22 @@ def main():
             plan.solve(tol, max_it, base, fetch_v=False)
         torch.cuda.synchronize()
         levels_value = len(nets) * max(1, args.steps // 2) / (time.perf_counter() - t0)
+        # its kernels one by one: algorithmic bytes over HIP-event time, per solve
+        plan.profile(True)
+        plan.solve(tol, max_it, base, fetch_v=False)
+        lrep = plan.profile_report()
+        plan.profile(False)
+        for name, v in lrep.items():
+            if not v["launches"]:
+                continue
+            gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 and v["bytes"] > 0 else None
+            level_kernels[name] = {"ms_per_solve": round(v["ms"], 3), "launches": v["launches"],
+                                   "achieved_GBps": None if gbs is None else round(gbs, 1),
+                                   "frac_hbm_peak": None if gbs is None else round(gbs / HBM_PEAK_GBS, 4)}
         plan.set_schedule("auto")
-        log(f"[bench] batched mixed, level schedule: {levels_value:.0f} solves/s")
+        log(f"[bench] batched mixed, level schedule: {levels_value:.0f} solves/s; "
+            + ", ".join(f"{k} {v['achieved_GBps']} GB/s" for k, v in level_kernels.items()
+                        if k in ("k_ybus", "k_mismatch", "k_jacobian", "k_update") and v["achieved_GBps"]))
 
     # accuracy of the mixed path against the FP64 path on the same batch
     r_mixed = plan.solve(tol, max_it, base, fetch_v=True)
@@ -185,6 +200,7 @@ def main():
         "plain_fp64_by_schedule": {k: round(v, 2) for k, v in single.items()},
         "speedup_vs_plain_fp64": round(value / plain, 2),
         "batched_mixed_level_schedule_solves_per_s": None if levels_value is None else round(levels_value, 1),
+        "level_schedule_kernels": level_kernels or None,
         "batched_fp64_solves_per_s": round(batched_fp64, 1),
         "iterations_p50": float(np.percentile(its, 50)), "iterations_p99": float(np.percentile(its, 99)),
         "mixed_vs_fp64_max_abs_dv": dv, "mixed_vs_fp64_max_iteration_diff": it_diff,
