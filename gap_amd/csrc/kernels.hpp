@@ -950,7 +950,7 @@ __global__ void __launch_bounds__(64) k_big_fwd_diag(FrontArgs a, int begin, int
 #pragma unroll
     for (int k = 0; k < kSolvePanel; ++k) {
         if (k < kb) {
-            T const yk = __shfl(wt, k);
+            T const yk = lane_value(wt, k);  // lane k's value, by readlane (no LDS round trip)
             if (t > k && t < kb) wt -= lr[k] * yk;
         }
     }
@@ -1022,7 +1022,7 @@ __global__ void __launch_bounds__(64) k_big_bwd_diag(FrontArgs a, int begin, int
 #pragma unroll
     for (int k = kSolvePanel - 1; k >= 0; --k) {
         if (k < kb) {
-            T const xk = __shfl(wt / diag, k);
+            T const xk = lane_value(wt, k) / lane_value(diag, k);  // lane k's quotient, by readlane
             if (t < k) wt -= ur[k] * xk;
             if (t == k) {
                 sol[cols[k]] = static_cast<double>(xk);
