@@ -325,6 +325,34 @@ __device__ inline TaskPre<T> load_pre(FusedArgs const& a, GridTask const& tk) {
     return r;
 }
 
+// Lane k of each group of L lanes, in every lane of the group.  Groups of 4
+// use a quad-permute DPP move (a VALU operation, no LDS round trip); larger
+// groups a bpermute shuffle from lane src = group base + k.  k is a
+// compile-time constant once the callers' loops are unrolled.
+template <int K>
+__device__ inline float quad_bcast(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, true));
+}
+template <int K>
+__device__ inline double quad_bcast(double v) {
+    int const lo = __builtin_amdgcn_mov_dpp(__double2loint(v), K * 0x55, 0xF, 0xF, true);
+    int const hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), K * 0x55, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+template <int L, typename T>
+__device__ inline T group_bcast(T v, int k, int src) {
+    if constexpr (L == 4) {
+        switch (k) {
+            case 0: return quad_bcast<0>(v);
+            case 1: return quad_bcast<1>(v);
+            case 2: return quad_bcast<2>(v);
+            default: return quad_bcast<3>(v);
+        }
+    } else {
+        return __shfl(v, src);
+    }
+}
+
 // Division by a pivot in the grid kernel's factorisation and solves: with
 // FP32 factors the hardware reciprocal (1 ulp, one instruction against a
 // dozen and two mode switches for the correctly rounded quotient), whose
@@ -443,8 +471,8 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
             int const src = gi * L + k;
             T lk[MM];
 #pragma unroll
-            for (int i = k; i < MM; ++i) lk[i] = __shfl(col[i], src);
-            T const wk = __shfl(wt, src);
+            for (int i = k; i < MM; ++i) lk[i] = group_bcast<L>(col[i], k, src);
+            T const wk = group_bcast<L>(wt, k, src);
             bool const act = k < p;
             T piv = act ? lk[k] : T(1);
             if (tabs(piv) < T(1e-14)) {
@@ -541,7 +569,7 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
         T acc = t < m ? w[t] : T(0);
         T xs[MM];
 #pragma unroll
-        for (int q = 0; q < MM; ++q) xs[q] = __shfl(acc, gi * L + q);
+        for (int q = 0; q < MM; ++q) xs[q] = group_bcast<L>(acc, q, gi * L + q);
         if (t < p) {
 #pragma unroll
             for (int q = 0; q < MM; ++q)
@@ -551,7 +579,7 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
 #pragma unroll
         for (int k = MM - 1; k >= 0; --k) {
             if (k < p_max) {
-                T const xk = __shfl(pivot_div(acc, urow[k]), gi * L + k);
+                T const xk = group_bcast<L>(pivot_div(acc, urow[k]), k, gi * L + k);
                 if (k < p) {
                     if (t < k) acc -= urow[k] * xk;
                     if (t == k) xme = xk;
