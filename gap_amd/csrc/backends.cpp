@@ -316,25 +316,33 @@ std::vector<BranchData> HipNewtonRaphson::branch_flows(NetworkData const& networ
 std::vector<Float> HipNewtonRaphson::calculate_mismatches(NetworkData const& network_data,
                                                           ComplexVector const& bus_voltages,
                                                           SparseMatrix const& admittance_matrix) {
-    // public entry of the reference: S_base fixed at 100 MVA, Y used as given,
-    // result in watts (cpu_newton_raphson.cpp:225-240)
+    // public entry of the reference: S_base fixed at 100 MVA, Y used as given
+    // whatever its pattern, result in watts / VAr (cpu_newton_raphson.cpp:225-240)
     double const base = 100e6;
-    flows_valid_ = false;  // the mismatch evaluation replaces the plan's voltages
-    plan_for(network_data);
     int const n = static_cast<int>(network_data.buses.size());
     if (static_cast<int>(bus_voltages.size()) != n) throw std::runtime_error("voltage vector size mismatch");
-    bool empty_y = admittance_matrix.nnz == 0 || admittance_matrix.values.empty();
-    if (!empty_y) give_y(admittance_matrix, 1);
+    // the reference's emptiness test (:301-305): zero calculated power
+    SparseMatrix const& y = admittance_matrix;
+    bool const empty_y = y.nnz == 0 || y.row_ptr.empty() || y.col_idx.empty() || y.values.empty();
+    int const rows = empty_y ? 0 : static_cast<int>(y.row_ptr.size()) - 1;
+    int const nnz = empty_y ? 0 : static_cast<int>(std::min(y.col_idx.size(), y.values.size()));
+    std::vector<double> yri(2 * static_cast<size_t>(nnz));
+    for (int k = 0; k < nnz; ++k) {
+        yri[2 * k] = y.values[k].real();
+        yri[2 * k + 1] = y.values[k].imag();
+    }
     std::vector<double> v(2 * static_cast<size_t>(n));
     for (int i = 0; i < n; ++i) {
         v[2 * i] = bus_voltages[i].real();
         v[2 * i + 1] = bus_voltages[i].imag();
     }
-    if (empty_y) std::fill(v.begin(), v.end(), 0.0);  // empty Y -> zero injections (:301-305)
+    NetArrays const arrays(network_data);
+    gaphip_net const view = arrays.view();
     std::vector<double> f(2 * static_cast<size_t>(n) + 1);
     int m = 0;
-    int rc = gaphip_mismatches(plan_, 0, v.data(), base, f.data(), static_cast<int>(f.size()), &m);
-    gaphip_plan_set_ybus(plan_, 0, nullptr, nullptr, nullptr, 0, 0);
+    int rc = gaphip_power_mismatches(&view, v.data(), rows, empty_y ? nullptr : y.row_ptr.data(),
+                                     empty_y ? nullptr : y.col_idx.data(), yri.data(), nnz, base, f.data(),
+                                     static_cast<int>(f.size()), &m);
     if (rc) raise(rc, "calculate_mismatches");
     std::vector<Float> out(m);
     for (int k = 0; k < m; ++k) out[k] = f[k] * base;

@@ -981,6 +981,7 @@ struct gaphip_plan {
     gaphip::DBuf<float> front32, vec32;
     gaphip::DevFronts fr;
     gaphip::DBuf<int> bus_off_d;  // per-instance bus offsets for the grid schedule
+    gaphip::DBuf<int> var_off_d;  // per-instance first Newton equation (n_inst + 1)
     int schedule = GAPHIP_SCHED_AUTO;
     gaphip_tuning tuning{2, 8, -1};
 
@@ -1261,6 +1262,7 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     GridItems items{it_i, it_j, it_y, it_pos};
     pack_fronts(plans, ent_base, key_row, key_col, P.n_var, P.j_nnz, P.fr, s, &items);
     P.bus_off_d.upload(P.bus_off, s);
+    P.var_off_d.upload(P.var_off, s);
     HIPCHK(hipStreamSynchronize(s));
     if (P.fr.fz_ok) {
         // the grid mismatch pass writes each -F where the factor tasks read
@@ -1422,7 +1424,7 @@ void run_solve(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front, T* v
         launch(P, K_MISMATCH, 20.0 * Z + 56.0 * N + 8.0 * V,
                [&] { hipLaunchKernelGGL(k_mismatch, dim3(blocks_for(N, 256)), dim3(256), 0, s, ma); });
         CheckArgs ca{G, it, o.tolerance, P.maxf.p + static_cast<size_t>(it) * G, P.done.p, P.iters.p,
-                     P.lu_fail.p, P.final_mis.p, P.active.p + it};
+                     P.lu_fail.p, P.final_mis.p, P.active.p + it, P.f.p, P.var_off_d.p};
         launch(P, K_CHECK, 24.0 * G, [&] { hipLaunchKernelGGL(k_check, dim3(blocks_for(G, 256)), dim3(256), 0, s, ca); });
         HIPCHK(hipMemcpyAsync(P.pinned_active, P.active.p + it, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHK(hipEventRecord(P.ev, s));
@@ -1532,7 +1534,7 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
         DevFronts const& d = P.fr;
         hipLaunchKernelGGL(k_gather_ypair, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
                            static_cast<int>(d.fz_npairs), d.fz_pairs.p, P.y_pu.p, d.fz_ypair.p);
-        FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
+        FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, P.var_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
                      d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_frhs.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
                      P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, P.dx.p};
         MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
@@ -2092,6 +2094,106 @@ int gaphip_mismatches(gaphip_plan* P, int inst, const double* v_ri, double base_
         *m_out = m;
         if (m > cap) throw Error(GAPHIP_E_CAPACITY, "mismatch capacity too small");
         if (m) HIPCHK(hipMemcpy(f, P->f.p + P->var_off[inst], sizeof(double) * m, hipMemcpyDeviceToHost));
+    })
+}
+
+int gaphip_power_mismatches(const gaphip_net* net, const double* v_ri, int y_rows, const int* y_row_ptr,
+                            const int* y_col_idx, const double* y_ri, int y_nnz, double base_power, double* f,
+                            int cap, int* m_out) {
+    GAPHIP_GUARD({
+        if (!net || !m_out || (net->n_bus > 0 && !v_ri)) throw Error(GAPHIP_E_ARG, "null argument");
+        if (!(base_power > 0.0)) throw Error(GAPHIP_E_ARG, "base_power must be > 0");
+        if (y_nnz < 0 || y_rows < 0) throw Error(GAPHIP_E_ARG, "negative admittance size");
+        if (y_nnz > 0 && (!y_row_ptr || !y_col_idx || !y_ri)) throw Error(GAPHIP_E_ARG, "missing admittance array");
+        int dev_count = 0;
+        if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+            throw Error(GAPHIP_E_DEVICE, "no HIP device visible");
+        Net const N = copy_net(*net);
+        int const n = N.n_bus();
+        // equation rows: P for every non-slack bus, then Q if PQ, in bus order
+        // (cpu_newton_raphson.cpp:269-289)
+        std::vector<int> peq(n, -1), qeq(n, -1);
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            if (N.bus_type[i] == kSlack) continue;
+            peq[i] = m++;
+            if (N.bus_type[i] == kPQ) qeq[i] = m++;
+        }
+        *m_out = m;
+        if (m > cap) throw Error(GAPHIP_E_CAPACITY, "mismatch capacity too small");
+        if (m > 0 && !f) throw Error(GAPHIP_E_ARG, "null output");
+        // The admittance matrix as the reference walks it (:296-337): rows past
+        // y_rows give zero current, a row stops at the end of the arrays and
+        // skips columns >= n_bus; y_nnz = 0 is the empty matrix (zero power).
+        std::vector<int> rp(n + 1, 0), ci;
+        std::vector<double2> yv;
+        for (int i = 0; i < n && y_nnz > 0; ++i) {
+            if (i < y_rows) {
+                for (int q = y_row_ptr[i]; q < y_row_ptr[i + 1] && q < y_nnz; ++q) {
+                    if (q < 0) throw Error(GAPHIP_E_ARG, "admittance row pointer out of range");
+                    int const k = y_col_idx[q];
+                    if (k < 0) throw Error(GAPHIP_E_ARG, "negative admittance column");
+                    if (k >= n) continue;
+                    ci.push_back(k);
+                    yv.push_back(make_double2(y_ri[2 * q], y_ri[2 * q + 1]));
+                }
+            }
+            rp[i + 1] = static_cast<int>(ci.size());
+        }
+        if (y_nnz <= 0) std::fill(rp.begin(), rp.end(), 0);
+        // specified power (:346-377): the bus's own injection when non-zero,
+        // plus every in-service SOURCE / LOADGEN appliance whose node equals the
+        // bus id, in appliance order, over base_power
+        std::vector<double2> sp(n);
+        for (int i = 0; i < n; ++i) {
+            double pr = 0.0, pi = 0.0;
+            if (N.p_w[i] != 0.0 || N.q_var[i] != 0.0) {
+                pr = N.p_w[i];
+                pi = N.q_var[i];
+            }
+            for (int a = 0; a < N.n_app(); ++a)
+                if (N.app_node[a] == N.bus_id[i] && N.app_status[a] == 1 &&
+                    (N.app_type[a] == kSource || N.app_type[a] == kLoadgen)) {
+                    pr += N.app_p[a];
+                    pi += N.app_q[a];
+                }
+            sp[i] = make_double2(pr / base_power, pi / base_power);
+        }
+        hipStream_t s;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } guard{s};
+        std::vector<double2> v(n);
+        for (int i = 0; i < n; ++i) v[i] = make_double2(v_ri[2 * i], v_ri[2 * i + 1]);
+        if (ci.empty()) {
+            ci.push_back(0);  // never read (every row is empty); keeps the buffers non-null
+            yv.push_back(make_double2(0.0, 0.0));
+        }
+        DBuf<int> d_rp, d_ci, d_peq, d_qeq, d_inst, d_done;
+        DBuf<double2> d_y, d_v, d_sp, d_s, d_vpol;
+        DBuf<double> d_f;
+        DBuf<unsigned long long> d_max;
+        d_rp.upload(rp, s);
+        d_ci.upload(ci, s);
+        d_y.upload(yv, s);
+        d_v.upload(v, s);
+        d_sp.upload(sp, s);
+        d_peq.upload(peq, s);
+        d_qeq.upload(qeq, s);
+        d_inst.upload(std::vector<int>(n, 0), s);
+        d_done.upload(std::vector<int>{0}, s);
+        d_max.upload(std::vector<unsigned long long>{0}, s);
+        d_s.alloc(n);
+        d_vpol.alloc(n);
+        d_f.alloc(std::max(m, 1));
+        MisArgs ma{n, d_rp.p, d_ci.p, d_y.p, d_v.p, d_sp.p, d_peq.p, d_qeq.p, d_inst.p, d_done.p, d_s.p, d_vpol.p,
+                   d_f.p, d_max.p};
+        if (n > 0) hipLaunchKernelGGL(k_mismatch, dim3(blocks_for(n, 256)), dim3(256), 0, s, ma);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        if (m) HIPCHK(hipMemcpy(f, d_f.p, sizeof(double) * m, hipMemcpyDeviceToHost));
     })
 }
 

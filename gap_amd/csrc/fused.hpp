@@ -43,6 +43,7 @@ struct FusedArgs {
     int order_off;        // this launch covers order[order_off + blockIdx.x]
     const int* order;     // networks, largest first
     const int* bus_off;   // per network, n_inst + 1 entries
+    const int* var_off;   // per network: first Newton equation / variable, n_inst + 1 entries
     const int* lvl_beg;   // per network: its levels are [lvl_beg[g], lvl_beg[g+1]) of lvl_ptr
     const int* lvl_ptr;   // per (network, level): tasks [lvl_ptr[e], lvl_ptr[e+1])
     const GridTask* tasks;
@@ -271,10 +272,9 @@ __device__ inline void grid_update_bus(UpdArgs const& a, int i, bool zero) {
         double const old = sqrt(v.x * v.x + v.y * v.y);
         double mc = 0.9 * (zero ? 0.0 : dxm);
         if (fabs(mc) > 0.08) mc = 0.08 * (mc > 0 ? 1.0 : -1.0);
-        double vm = old + mc;
-        vm = fmax(vm, 0.5);
-        vm = fmin(vm, 1.5);
-        v = old > 0.0 ? make_double2(v.x * (vm / old), v.y * (vm / old)) : make_double2(vm, 0.0);
+        double const vm = ref_clamp_magnitude(old + mc);
+        // a NaN magnitude scales V to NaN, as the reference's polar rebuild does
+        v = !(old <= 0.0) ? make_double2(v.x * (vm / old), v.y * (vm / old)) : make_double2(vm, 0.0);
     }
     a.v[i] = v;
 }
@@ -705,6 +705,9 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1)
     int const tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     T* pool = reinterpret_cast<T*>(grid_smem) + wave * a.pool;
     for (int it = 0; it < a.max_it; ++it) {
+        // cleared before block_max's barriers, which order it before any
+        // factor task of this iteration can set it
+        if (tid == 0) a.lu_fail[g] = 0;
         double m = 0.0;
         constexpr int MB = sizeof(T) == 8 ? 2 : 1;  // buses per thread and step
         for (int i = b0 + tid; i < b1; i += MB * NT) {
@@ -714,10 +717,10 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1)
             m = fmax(m, grid_mismatch_buses<T, MB>(ma, ib, i + (MB - 1) * NT < b1));
         }
         m = block_max<NW>(m, red);  // its barriers also publish S and F
+        if (int const r0 = a.var_off[g]; r0 < a.var_off[g + 1]) m = reference_max(m, ma.f[r0]);
         if (tid == 0) {
             a.maxf[static_cast<long long>(it) * a.n_inst + g] = dbits(m);
             a.final_mis[g] = m;
-            a.lu_fail[g] = 0;
         }
         if (m < a.tol) {
             if (tid == 0) {

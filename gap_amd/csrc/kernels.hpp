@@ -12,6 +12,23 @@ namespace gaphip {
 
 constexpr int kWave = 64;
 
+// The reference's max|F| (cpu_newton_raphson.cpp:115-119) is max_element
+// under `|a| < |b|`: a NaN in the first mismatch row is kept (no comparison
+// with it is true), a NaN in any later row is never taken.  The kernels reduce
+// with fmax, which skips NaN, and then apply the first row: `max_nonnan` is
+// the NaN-free maximum, `f0` the instance's first mismatch row.  A NaN result
+// fails `max < tol`, so such a network is not reported converged.
+__device__ inline double reference_max(double max_nonnan, double f0) { return f0 != f0 ? fabs(f0) : max_nonnan; }
+
+// |V| clamped to [0.5, 1.5] as std::max(v, 0.5) then std::min(v, 1.5)
+// (cpu_newton_raphson.cpp:493-496): each returns its first argument unless
+// the comparison holds, so a NaN magnitude stays NaN (fmax / fmin would
+// replace it by the bound).
+__device__ inline double ref_clamp_magnitude(double vm) {
+    vm = (vm < 0.5) ? 0.5 : vm;
+    return (1.5 < vm) ? 1.5 : vm;
+}
+
 // ------------------------------------------------------------------ Y-bus
 // One thread per CSR slot walks its contribution list in the reference order
 // (shunts, then branches; cpu_admittance_matrix.cpp:86-154).  Contraction is
@@ -183,6 +200,7 @@ __global__ void __launch_bounds__(256) k_mismatch(MisArgs a) {
         for (int off = kWave / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
         if (lane == 0 && inst0 >= 0 && m > 0.0) atomicMax(&a.maxf[inst0], dbits(m));
     } else if (live && m > 0.0) {
+        // fabs clears the sign, so a NaN's bits lie above +inf's and win the max
         atomicMax(&a.maxf[inst], dbits(m));
     }
 }
@@ -192,18 +210,25 @@ struct CheckArgs {
     int n_inst;
     int iter;
     double tol;
-    const unsigned long long* maxf;  // this iteration
+    unsigned long long* maxf;  // this iteration (its reference maximum is written back)
     int* done;
     int* iters;
     int* lu_fail;
     double* final_mismatch;
     int* active;  // this iteration's counter of instances still iterating
+    const double* f;        // this iteration's mismatch vector
+    const int* var_off;     // per instance: its first mismatch row (n_inst + 1 entries)
 };
 __global__ void __launch_bounds__(256) k_check(CheckArgs a) {
     int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.n_inst) return;
     if (a.done[g]) return;
     double m = __longlong_as_double(static_cast<long long>(a.maxf[g]));
+    int const r0 = a.var_off[g];
+    if (r0 < a.var_off[g + 1]) {
+        m = reference_max(m, a.f[r0]);
+        a.maxf[g] = dbits(m);  // the trajectory row holds the reference's maximum
+    }
     a.final_mismatch[g] = m;
     a.lu_fail[g] = 0;
     if (m < a.tol) {
@@ -1359,8 +1384,7 @@ __device__ inline void update_bus(UpdArgs const& a, int i, bool zero) {
         if (fabs(mc) > 0.08) mc = 0.08 * (mc > 0 ? 1.0 : -1.0);
         double vm = old + mc;
         double va = atan2(v.y, v.x);
-        vm = fmax(vm, 0.5);
-        vm = fmin(vm, 1.5);
+        vm = ref_clamp_magnitude(vm);
         v = make_double2(vm * cos(va), vm * sin(va));
     }
     a.v[i] = v;

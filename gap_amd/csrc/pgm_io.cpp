@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <iostream>
 #include <map>
 #include <numeric>
 #include <sstream>
@@ -371,6 +372,9 @@ NetworkData network_from_rows(Rows const& rows) {
 NetworkData PgmJsonIO::read_network_data(std::string const& filename) {
     rows_ = rows_from_pgm(parse_json(read_file(filename)), base_power_);
     last_ = network_from_rows(rows_);
+    // as the reference reader (json_io.cpp:496-499): a warning, not an error
+    if (!last_.validate_pgm_compliance())
+        std::cerr << "warning: " << filename << ": network data does not meet PGM compliance standards\n";
     return last_;
 }
 

@@ -215,7 +215,11 @@ PYBIND11_MODULE(gap_solver, m) {
         .def_readwrite("num_appliances", &NetworkData::num_appliances)
         .def_readwrite("version", &NetworkData::version)
         .def_readwrite("type", &NetworkData::type)
-        .def_readwrite("is_batch", &NetworkData::is_batch);
+        .def_readwrite("is_batch", &NetworkData::is_batch)
+        .def("validate_pgm_compliance", &NetworkData::validate_pgm_compliance,
+             "True when every bus has u_rated > 0 and every branch r1 >= 0, x1 != 0, i_n > 0");
+    m.def("appliance_type_to_string", &appliance_type_to_string, py::arg("type"));
+    m.def("bus_type_to_string", &bus_type_to_string, py::arg("type"));
     py::class_<SparseMatrix>(m, "SparseMatrix")
         .def(py::init<>())
         .def_readwrite("row_ptr", &SparseMatrix::row_ptr)

@@ -93,6 +93,18 @@ struct NetworkData {
     std::string version{"1.0"};
     std::string type{"input"};
     bool is_batch{false};
+
+    // The reference's consistency check (types.h:337-351 of the reference,
+    // called by its JSON reader and bound in Python): every bus has a positive
+    // rated voltage and every branch r1 >= 0, x1 != 0 and a positive rated
+    // current.  Appliances are not checked.
+    bool validate_pgm_compliance() const {
+        for (BusData const& b : buses)
+            if (b.u_rated <= 0.0) return false;
+        for (BranchData const& br : branches)
+            if (br.r1 < 0.0 || br.x1 == 0.0 || br.i_n <= 0.0) return false;
+        return true;
+    }
 };
 
 // GPU_HIP fills the slot the reference left commented out (types.h:357).
@@ -103,6 +115,15 @@ inline std::string backend_type_to_string(BackendType t) {
         case BackendType::CPU: return "CPU";
         case BackendType::GPU_CUDA: return "GPU_CUDA";
         case BackendType::GPU_HIP: return "GPU_HIP";
+    }
+    return "Unknown";
+}
+
+inline std::string appliance_type_to_string(ApplianceType t) {
+    switch (t) {
+        case ApplianceType::SOURCE: return "SOURCE";
+        case ApplianceType::LOADGEN: return "LOADGEN";
+        case ApplianceType::SHUNT: return "SHUNT";
     }
     return "Unknown";
 }

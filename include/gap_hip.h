@@ -203,6 +203,20 @@ int gaphip_jacobian(gaphip_plan* plan, int inst, const double* v_ri, double base
 int gaphip_mismatches(gaphip_plan* plan, int inst, const double* v_ri, double base_power,
                       double* f, int cap, int* m_out);
 
+/* Mismatch vector of one network at v_ri (2*n_bus) for a caller-given
+ * admittance matrix of any pattern, used as given (no per-unit scaling),
+ * with S_spec over base_power: the evaluation behind the public
+ * IPowerFlowSolver::calculate_mismatches (include/gap/solver/powerflow_interface.h:46-48;
+ * cpu_newton_raphson.cpp:225-337, which multiplies the result by its fixed
+ * 100 MVA).  y_rows = row_ptr entries - 1; y_nnz = the shorter of the column
+ * and value arrays.  As the reference walks the matrix: rows past y_rows and
+ * an empty matrix (y_nnz = 0) give zero current, a row stops at the end of
+ * the arrays, columns >= n_bus are skipped (negative ones are E_ARG).
+ * Rows: P of every non-slack bus, then Q if PQ, in bus order; *m_out is set. */
+int gaphip_power_mismatches(const gaphip_net* net, const double* v_ri, int y_rows, const int* y_row_ptr,
+                            const int* y_col_idx, const double* y_ri, int y_nnz, double base_power, double* f,
+                            int cap, int* m_out);
+
 /* Adjoint sensitivities at the converged state of the last solve (FP64):
  * solves J^T lambda = dL/dx with the factors of J at the solution and returns
  * dL/d(bus p_w, bus q_var) [per W / VAr] and dL/d(branch r1, x1) [per ohm] for

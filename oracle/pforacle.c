@@ -860,9 +860,12 @@ int pfo_solve(const pf_net* net, double tol, int max_it, double base_power, doub
     for (int it = 0; it < max_it; ++it) {
         power_injections(&y, vr, vi, sr, si);
         int m = mismatch(net, sr, si, pr, pi, f);
-        double mx = 0.0; /* first maximum of |F| (:115-120) */
-        for (int k = 0; k < m; ++k)
-            if (fabs(f[k]) > mx) mx = fabs(f[k]);
+        /* |F| of the first maximum under max_element's `|a| < |b|` test
+         * (:115-120): a NaN in row 0 stays the maximum (no comparison with
+         * it is true), a NaN in a later row is never taken */
+        double mx = m > 0 ? fabs(f[0]) : 0.0;
+        for (int k = 1; k < m; ++k)
+            if (mx < fabs(f[k])) mx = fabs(f[k]);
         fm = mx;
         if (traj && nt < traj_cap) traj[nt] = mx;
         ++nt;

@@ -28,6 +28,27 @@ def golden_cases(prefix=""):
     return sorted(p.stem for p in GOLDEN.glob(f"{prefix}*.npz") if not p.stem.startswith("lu_"))
 
 
+def finite_golden_cases():
+    """Golden cases whose reference solution is finite (not the NaN-load ones)."""
+    return [n for n in golden_cases() if not n.startswith("fail_nan")]
+
+
+def max_dv(a, b):
+    """max |a - b| over complex arrays; an entry NaN in both counts as equal,
+    NaN in only one of them as infinitely far."""
+    a, b = np.asarray(a), np.asarray(b)
+    na, nb = np.isnan(a), np.isnan(b)
+    if np.any(na != nb):
+        return np.inf
+    d = np.abs(np.where(na, 0, a - b))
+    return float(d.max()) if d.size else 0.0
+
+
+def same_nan(a, b):
+    """True when a and b are NaN at the same positions."""
+    return bool(np.array_equal(np.isnan(np.asarray(a)), np.isnan(np.asarray(b))))
+
+
 def load_golden(name):
     from gap_amd.network import Network
     d = np.load(GOLDEN / f"{name}.npz", allow_pickle=False)

@@ -14,7 +14,7 @@ settings, the reference Y-bus (CSR + values), the reference Jacobian of the
 first Newton iteration (pattern + values), the per-iteration max|F|, the
 iteration count, the convergence flag and the final voltages.
 
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [--failure-only]
 """
 from __future__ import annotations
 
@@ -94,6 +94,30 @@ def special_cases():
     return cases
 
 
+def failure_cases():
+    """The reference's failure semantics (cpu_newton_raphson.cpp:115-157, 382-433):
+    a NaN load (in a later mismatch row, which max_element never takes, and in
+    the first row, which it always keeps), and a PQ bus with a load whose every
+    branch is out of service (its Jacobian rows and columns are exactly zero and
+    dropped by the 1e-12 compaction, the factorisation fails and every
+    correction is zero, so the iteration runs to the cap)."""
+    cases = {}
+    base = gridgen.small_mesh(20, seed=4)
+    for name, bus in (("fail_nan_load_later_row", 7), ("fail_nan_load_first_row", 1)):
+        n = base.copy()
+        p = n.p_w.copy()
+        p[bus] = np.nan
+        n.p_w = p
+        cases[name] = (n, 100e6, 1e-8, 6)
+    n = base.copy()
+    st = n.br_status.copy()
+    iso = int(n.bus_id[13])
+    st[(n.br_from == iso) | (n.br_to == iso)] = 0
+    n.br_status = st
+    cases["fail_isolated_pq_bus"] = (n, 100e6, 1e-8, 6)
+    return cases
+
+
 def run_case(R, name, net, base, tol, max_it, extra=None):
     t = time.time()
     r = R.solve(net, tol=tol, max_it=max_it, base_power=base, j_call=0)
@@ -115,6 +139,14 @@ def run_case(R, name, net, base, tol, max_it, extra=None):
 def main():
     pyoracle.build(with_ref=True)
     R = pyoracle.Reference()
+    if "--failure-only" in sys.argv:
+        manifest = json.load(open(OUT / "manifest.json"))
+        for name, (net, base, tol, mi) in failure_cases().items():
+            run_case(R, name, net, base, tol, mi)
+            manifest[name] = {"generator": "make_golden.failure_cases", "base_power": base}
+        with open(OUT / "manifest.json", "w") as f:
+            json.dump(manifest, f, indent=1)
+        return
     manifest = {}
     for case in PGM_CASES:
         d = PGM_DIR / case
@@ -134,6 +166,9 @@ def main():
     for name, (net, base, tol, mi) in special_cases().items():
         run_case(R, name, net, base, tol, mi)
         manifest[name] = {"generator": "make_golden.special_cases", "base_power": base}
+    for name, (net, base, tol, mi) in failure_cases().items():
+        run_case(R, name, net, base, tol, mi)
+        manifest[name] = {"generator": "make_golden.failure_cases", "base_power": base}
     # LU known answers on seeded diagonally dominant complex matrices (the shape of
     # tests/validation/lu_solver_validation.cpp:23-105), solved by the reference LU
     rng = np.random.default_rng(12345)

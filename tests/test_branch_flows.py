@@ -13,7 +13,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from conftest import golden_cases, load_golden
+from conftest import finite_golden_cases, golden_cases, load_golden
 from flows_ref import FIELDS, branch_flows
 from gap_amd import capi, gridgen
 from gap_amd.network import SHUNT
@@ -78,7 +78,7 @@ def test_out_of_service_branch_is_zero():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", golden_cases())
+@pytest.mark.parametrize("name", finite_golden_cases())
 def test_gpu_branch_flows_vs_checker(device, name):
     net, g = load_golden(name)
     base = float(g["base_power"])

@@ -4,7 +4,7 @@ what the mathematics fixes independently of both."""
 import numpy as np
 import pytest
 
-from conftest import golden_cases, load_golden, GOLDEN
+from conftest import golden_cases, load_golden, max_dv, same_nan, GOLDEN
 from gap_amd import gridgen
 from gap_amd.network import Network, PQ, SLACK
 
@@ -26,14 +26,16 @@ def test_nr_matches_reference(oracle, name):
                      base_power=float(g["base_power"]), j_call=0)
     assert r["iterations"] == int(g["iterations"])
     assert r["converged"] == bool(g["converged"])
-    assert np.abs(r["v"] - g["v"]).max() <= 1e-10
+    assert max_dv(r["v"], g["v"]) <= 1e-10
+    assert same_nan(r["final_mismatch"], g["final_mismatch"])
     # Jacobian of the first iteration: pattern bit-exact, values as computed
     jrp, jci, jv = r["J"]
     assert np.array_equal(jrp, g["j_rp"]) and np.array_equal(jci, g["j_ci"])
     assert np.allclose(jv, g["j_val"], rtol=1e-13, atol=1e-13)
     # the damped iteration contracts |F| by ~0.1; early entries agree tightly
     k = min(4, len(g["traj"]))
-    assert np.allclose(r["traj"][:k], g["traj"][:k], rtol=1e-9)
+    assert np.allclose(r["traj"][:k], g["traj"][:k], rtol=1e-9, equal_nan=True)
+    assert len(r["traj"]) == len(g["traj"]) and same_nan(r["traj"], g["traj"])
 
 
 @pytest.mark.parametrize("name", golden_cases("pgm_"))
@@ -120,18 +122,12 @@ def test_mismatch_closed_forms(oracle):
     assert np.allclose(f0, [2.5e5, 1e5])
 
 
-def test_jacobian_equals_finite_difference(oracle):
-    """The analytic Jacobian against central differences of the mismatch in the
-    reference variables (angles of non-slack buses, then |V| of PQ buses)."""
-    net = gridgen.small_mesh(9, seed=4)
-    rng = np.random.default_rng(0)
-    vm = rng.uniform(0.95, 1.05, net.n_bus)
-    va = rng.uniform(-0.05, 0.05, net.n_bus)
-    v = vm * np.exp(1j * va)
+def _jacobian_and_fd(oracle, net):
+    """The oracle's first-iteration Jacobian (flat start, dense) and central
+    differences of its mismatch in the reference variables (angles of non-slack
+    buses, then |V| of PQ buses), with the tolerance the two must meet."""
     y = oracle.build_ybus(net)
     ypu = (y[0], y[1], y[2] * (10e3 ** 2 / 100e6))
-    # J at v, restated through pfo_jacobian via a solve capture is awkward; use the
-    # first-iteration capture at flat start instead and difference around flat start
     r = oracle.solve(net, max_it=1, j_call=0)
     jrp, jci, jv = r["J"]
     n_var = len(jrp) - 1
@@ -155,21 +151,34 @@ def test_jacobian_equals_finite_difference(oracle):
         e = np.zeros(n_var)
         e[k] = h
         Jfd[:, k] = (F(e) - F(-e)) / (2 * h)
-    assert np.allclose(J, Jfd, atol=1e-6 * np.abs(J).max())
-    del v, vm, va
+    return J, Jfd, 1e-6 * np.abs(J).max()
 
 
-def test_planted_sign_error_is_caught(oracle):
-    """A plausible slip (dropping the sign of the Q-angle diagonal) must fail the
-    finite-difference pin: restate the diagonal with the wrong sign and compare."""
-    net = gridgen.small_mesh(6, seed=2)
-    r = oracle.solve(net, max_it=1, j_call=0)
-    jrp, jci, jv = r["J"]
-    bad = jv.copy()
-    # second row is the Q row of the first non-slack PQ bus; its angle diagonal
-    # is entry (1, 0); flip it
-    row = 1
-    for q in range(jrp[row], jrp[row + 1]):
-        if jci[q] == 0:
-            bad[q] = -bad[q]
-    assert not np.allclose(bad, jv)
+def test_jacobian_equals_finite_difference(oracle):
+    """The analytic Jacobian against central differences of the mismatch."""
+    J, Jfd, atol = _jacobian_and_fd(oracle, gridgen.small_mesh(9, seed=4))
+    assert np.allclose(J, Jfd, atol=atol)
+
+
+@pytest.mark.parametrize("entry", ["q_angle_diag", "p_mag_offdiag", "dropped_term"])
+def test_planted_error_is_caught(oracle, entry):
+    """Plausible slips planted in the oracle's Jacobian (a flipped sign on the
+    Q-angle diagonal, a flipped off-diagonal dP/d|V|, a dropped diagonal term
+    -B|V|^2 of dP/dtheta) must fail the same finite-difference check that the
+    correct Jacobian passes."""
+    net = gridgen.small_mesh(9, seed=4)
+    J, Jfd, atol = _jacobian_and_fd(oracle, net)
+    assert np.allclose(J, Jfd, atol=atol)
+    bad = J.copy()
+    n_ang = int((net.bus_type != SLACK).sum())
+    if entry == "q_angle_diag":
+        bad[1, 0] = -bad[1, 0]            # row 1: Q of the first PQ bus; column 0: its angle
+    elif entry == "p_mag_offdiag":
+        r, c = next((r, c) for r in range(0, n_ang, 1) for c in range(n_ang, J.shape[1])
+                    if J[r, c] != 0.0 and c - n_ang != r)
+        bad[r, c] = -bad[r, c]
+    else:
+        y = oracle.build_ybus(net)
+        b00 = (y[2] * (10e3 ** 2 / 100e6))[y[0][1]:y[0][2]][list(y[1][y[0][1]:y[0][2]]).index(1)].imag
+        bad[0, 0] += b00 * 0.98 ** 2     # J[0,0] = -Q - B V^2 with the B V^2 term dropped
+    assert not np.allclose(bad, Jfd, atol=atol)
