@@ -15,6 +15,13 @@ with the same backward (the loss also depends on r, x explicitly).
 
 Tensors may live on any device; the plan's device buffers are the compute
 path, tensors only carry values in and gradients out.
+
+The adjoint belongs to the solve that produced the outputs.  ``PowerFlow``
+keeps its inputs and the plan's epoch (solves and input changes); if the plan was solved again or
+its inputs changed before ``backward`` runs, backward re-loads those inputs and
+solves again before taking the adjoint, so interleaved forwards on one plan
+give each output its own gradients.  ``PowerFlowLoss`` takes its gradients in
+forward.
 """
 from __future__ import annotations
 
@@ -38,7 +45,7 @@ def _np(t: torch.Tensor) -> np.ndarray:
 
 
 def _load(plan: Plan, p_w, q_var, r, x):
-    p, q, rr, xx = _np(p_w), _np(q_var), _np(r), _np(x)
+    p, q, rr, xx = (a if isinstance(a, np.ndarray) else _np(a) for a in (p_w, q_var, r, x))
     bo = 0
     ro = 0
     for k, net in enumerate(plan.nets):
@@ -67,6 +74,9 @@ class PowerFlow(torch.autograd.Function):
         _load(plan, p_w, q_var, r, x)
         v = _solve(plan, opts)
         ctx.plan = plan
+        ctx.opts = opts
+        ctx.epoch = plan.epoch
+        ctx.inputs = (_np(p_w).copy(), _np(q_var).copy(), _np(r).copy(), _np(x).copy())
         ctx.dev = p_w.device
         vm = torch.as_tensor(np.abs(v), device=p_w.device)
         va = torch.as_tensor(np.angle(v), device=p_w.device)
@@ -76,6 +86,10 @@ class PowerFlow(torch.autograd.Function):
     def backward(ctx, g_vm, g_va):
         gvm = None if g_vm is None else _np(g_vm)
         gva = None if g_va is None else _np(g_va)
+        if ctx.plan.epoch != ctx.epoch:
+            # the plan solved other inputs since this forward: restore ours
+            _load(ctx.plan, *ctx.inputs)
+            _solve(ctx.plan, ctx.opts)
         out = ctx.plan.adjoint(0.0, 0.0, gvm, gva)
         d = ctx.dev
         return (torch.as_tensor(out["grad_p"], device=d), torch.as_tensor(out["grad_q"], device=d),

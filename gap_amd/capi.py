@@ -208,6 +208,9 @@ class Plan:
         self._keep = None
         self.bus_off = np.concatenate([[0], np.cumsum([n.n_bus for n in self.nets])]).astype(np.int64)
         self.schedule = self.set_schedule(schedule)
+        # counts solves and input changes: a result taken from the plan belongs
+        # to the epoch it was read in (the autograd wrapper checks it)
+        self.epoch = 0
 
     def set_schedule(self, schedule: int | str = SCHED_AUTO) -> str:
         """Choose the device schedule ('auto', 'levels', 'grid'); returns the one solves will use."""
@@ -256,6 +259,7 @@ class Plan:
         q = np.ascontiguousarray(q_var, np.float64)
         if len(p) != self.nets[inst].n_bus or len(q) != self.nets[inst].n_bus:
             raise ValueError("injection arrays must have n_bus entries")
+        self.epoch += 1
         _check(lib().gaphip_plan_set_injections(self._h, inst, _p(p, _d), _p(q, _d)), "set_injections")
 
     def set_branch_rx(self, inst: int, r, x):
@@ -263,12 +267,14 @@ class Plan:
         x = np.ascontiguousarray(x, np.float64)
         if len(r) != self.nets[inst].n_branch or len(x) != self.nets[inst].n_branch:
             raise ValueError("r / x arrays must have n_branch entries")
+        self.epoch += 1
         _check(lib().gaphip_plan_set_branch_rx(self._h, inst, _p(r, _d), _p(x, _d)), "set_branch_rx")
 
     def solve(self, tolerance=1e-8, max_iterations=100, base_power=100e6, fetch_v=True):
         G = len(self.nets)
         o = SolveOpts(tolerance, int(max_iterations), base_power)
         v = np.zeros(2 * int(self.bus_off[-1])) if fetch_v else None
+        self.epoch += 1
         it = np.zeros(G, np.int32)
         cv = np.zeros(G, np.int32)
         fm = np.zeros(G)
@@ -332,6 +338,7 @@ class Plan:
         x = np.ascontiguousarray(x1, np.float64)
         if not (len(b) == len(r) == len(x)):
             raise ValueError("branch, r1 and x1 differ in length")
+        self.epoch += 1
         _check(lib().gaphip_plan_update_branches(self._h, inst, len(b), _p(b, _i), _p(r, _d), _p(x, _d)),
                "update_branches")
         for k, bi in enumerate(b):

@@ -995,6 +995,9 @@ struct gaphip_plan {
     int maxf_rows = 0;
     std::vector<unsigned long long> maxf_host;
     bool have_solution = false;
+    bool solution_converged = false;  // every network of the last solve converged
+    bool inputs_changed = false;      // injections, branch data or Y-bus changed since the last solve
+    bool branches_changed = false;    // branch r / x or Y-bus changed since the last solve
     bool y_assembled = false;  // y_pu holds the Y-bus of the last solve's base
     std::vector<std::vector<int>> br_slot_ptr, br_slot;  // per instance: Y slots each branch touches (lazy)
     gaphip::DBuf<int> upd_idx, upd_slots;
@@ -1594,6 +1597,10 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
 void run_adjoint(gaphip_plan& P, double w_loss, double w_dev, double const* g_vm, double const* g_va,
                  double* gp, double* gq, double* gr, double* gx, double* loss_out) {
     if (!P.have_solution) throw Error(GAPHIP_E_STATE, "adjoint needs a converged solve first");
+    if (P.inputs_changed)
+        throw Error(GAPHIP_E_STATE, "plan inputs changed since the last solve: solve again before the adjoint");
+    if (!P.solution_converged)
+        throw Error(GAPHIP_E_STATE, "adjoint needs a solve in which every network converged");
     hipStream_t s = P.stream;
     int const G = P.n_inst;
     long long const N = P.n_bus;
@@ -1680,6 +1687,7 @@ void run_update_branches(gaphip_plan& P, int g, int n, const int* branch, const 
     int const m = N.n_branch();
     for (int t = 0; t < n; ++t)
         if (branch[t] < 0 || branch[t] >= m) throw Error(GAPHIP_E_ARG, "branch index out of range");
+    P.inputs_changed = P.branches_changed = true;
     if (P.br_slot_ptr.empty()) {
         P.br_slot_ptr.resize(P.n_inst);
         P.br_slot.resize(P.n_inst);
@@ -1745,6 +1753,8 @@ void run_get_ybus(gaphip_plan const& P, int g, int* rp, int* ci, double* val_ri,
 // Branch flows at the last solution, kFlowFields per branch into host `out`.
 void run_branch_flows(gaphip_plan& P, double* out) {
     if (!P.have_solution) throw Error(GAPHIP_E_STATE, "branch flows need a solve first");
+    if (P.branches_changed)
+        throw Error(GAPHIP_E_STATE, "branch data changed since the last solve: solve again before the flows");
     if (!out) throw Error(GAPHIP_E_ARG, "null output");
     if (P.n_br == 0) return;
     hipStream_t s = P.stream;
@@ -1867,6 +1877,7 @@ int gaphip_plan_set_injections(gaphip_plan* P, int inst, const double* p_w, cons
         N.p_w.assign(p_w, p_w + N.n_bus());
         N.q_var.assign(q_var, q_var + N.n_bus());
         P->sspec_dirty = true;
+        P->inputs_changed = true;
     })
 }
 
@@ -1877,6 +1888,7 @@ int gaphip_plan_set_branch_rx(gaphip_plan* P, int inst, const double* r, const d
         int const m = N.n_branch();
         N.br_r.assign(r, r + m);
         N.br_x.assign(x, x + m);
+        P->inputs_changed = P->branches_changed = true;
         if (m) {
             HIPCHK(hipMemcpyAsync(P->br_r.p + P->br_off[inst], r, sizeof(double) * m, hipMemcpyHostToDevice,
                                   P->stream));
@@ -1892,6 +1904,7 @@ int gaphip_plan_set_ybus(gaphip_plan* P, int inst, const int* rp, const int* ci,
     GAPHIP_GUARD({
         if (!P || inst < 0 || inst >= P->n_inst) throw Error(GAPHIP_E_ARG, "bad argument");
         if (P->y_given_mode.empty()) P->y_given_mode.assign(P->n_inst, 0);
+        P->inputs_changed = P->branches_changed = true;
         if (!rp) {
             P->y_given_mode[inst] = 0;
             return GAPHIP_OK;
@@ -1960,6 +1973,8 @@ int gaphip_solve(gaphip_plan* P, const gaphip_solve_opts* o, double* v_ri, int* 
         HIPCHK(hipMemcpy(its.data(), P->iters.p, sizeof(int) * G, hipMemcpyDeviceToHost));
         for (int g = 0; g < G; ++g)
             if (!done[g]) its[g] = o->max_iterations;
+        P->solution_converged = std::all_of(done.begin(), done.end(), [](int d) { return d != 0; });
+        P->inputs_changed = P->branches_changed = false;
         if (iterations) std::copy(its.begin(), its.end(), iterations);
         if (converged) std::copy(done.begin(), done.end(), converged);
         if (final_mismatch) HIPCHK(hipMemcpy(final_mismatch, P->final_mis.p, sizeof(double) * G, hipMemcpyDeviceToHost));

@@ -222,7 +222,11 @@ int gaphip_power_mismatches(const gaphip_net* net, const double* v_ri, int y_row
  * dL/d(bus p_w, bus q_var) [per W / VAr] and dL/d(branch r1, x1) [per ohm] for
  *   L = w_loss * (total active branch loss, pu) + w_dev * sum_PQ(|V_i| - 1)^2
  *       + sum_i g_vm[i] |V_i| + sum_i g_va[i] arg V_i          (g_* may be NULL).
- * Arrays are concatenated over instances.  loss_out (n_inst) may be NULL. */
+ * Arrays are concatenated over instances.  loss_out (n_inst) may be NULL.
+ * GAPHIP_E_STATE when no solve has run, when a network of the last solve did
+ * not converge, or when injections, branch data or the Y-bus were changed
+ * (set_injections, set_branch_rx, update_branches, set_ybus) after it: the
+ * gradients belong to a converged solve of the current inputs. */
 int gaphip_adjoint(gaphip_plan* plan, double w_loss, double w_dev, const double* g_vm,
                    const double* g_va, double* grad_p, double* grad_q, double* grad_r,
                    double* grad_x, double* loss_out);
@@ -237,7 +241,8 @@ int gaphip_adjoint(gaphip_plan* plan, double w_loss, double w_dev, const double*
  * network's per-unit system; currents in A of its voltage base u_rated[0].
  * Out-of-service branches get zeros; `loading` needs a rating BranchData does
  * not carry and is not computed.  out: 10 * total branches doubles.
- * Before any solve: GAPHIP_E_STATE. */
+ * Before any solve, or after a change of branch data or the Y-bus since the
+ * last solve: GAPHIP_E_STATE. */
 #define GAPHIP_FLOW_FIELDS 10
 int gaphip_branch_flows(gaphip_plan* plan, double* out);
 

@@ -25,7 +25,8 @@ def oracle():
 
 
 def golden_cases(prefix=""):
-    return sorted(p.stem for p in GOLDEN.glob(f"{prefix}*.npz") if not p.stem.startswith("lu_"))
+    """Single-network golden solves (not the LU vectors or the oracle fixtures)."""
+    return sorted(p.stem for p in GOLDEN.glob(f"{prefix}*.npz") if not p.stem.startswith(("lu_", "batch")) and not p.stem.endswith("_oracle"))
 
 
 def finite_golden_cases():

@@ -186,51 +186,75 @@ def test_adjoint_vs_finite_differences(device, oracle, order, h_rel, bar):
     assert errs.max() < bar
 
 
-def test_adjoint_10k_grid_scenarios(device, oracle):
-    """10k-bus meshed grid (1,500 transformers, 300 shunts, 15% PV) x 4 load
-    scenarios (+-30%) in one plan: the FP64 adjoint against the sparse FP64
-    reference gradient (scenario 0: every load, 300 random branches) and
-    against eighth-order central differences of full re-solves on 16
-    parameters of scenario 1."""
+def test_adjoint_10k_x256(device, oracle):
+    """The adjoint workload at full size: the 10k-bus meshed grid (1,500
+    transformers, 300 shunts, 15% PV) x 256 load scenarios (+-30%) in one plan,
+    L = total active loss + sum over PQ buses of (|V| - 1)^2, FP64 adjoint of all
+    256 scenarios in one call.  Checked against
+      * the sparse FP64 reference gradient (tests/adjoint_ref.py) on scenarios 0
+        and 137: every load p, q and 400 random branches r, x, to 1e-6 of the
+        largest gradient of each kind;
+      * central differences of full re-solves on scenario 137, eighth order at
+        h = 3e-2 relative, on the largest gradients of each kind (where a
+        difference quotient resolves 1e-6) and on random parameters; and the
+        second-order stencil at h = 1e-6 on the largest branch gradients.  Each
+        parameter is judged against 1e-6 |g| plus three times its noise bound
+        (the loss of a solve carries round-off of ~5e-14, measured on copies
+        whose loads differ in the 13th digit, tests/fd_check.py); the resolvable
+        ones must be within 1e-6."""
+    import sys
+    import time
     from adjoint_ref import reference_gradient
+    from fd_check import GRAD_KEY, assess, fd_gradients, noise_floor
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
     base = 100e6
     net = gridgen.large_mesh(10_000, seed=1)
-    scen = gridgen.load_scenarios(net, 4, seed=2000)
+    scen = gridgen.load_scenarios(net, 256, seed=2000)
+    nb, nr = net.n_bus, net.n_branch
+    t0 = time.time()
     plan = capi.Plan(scen)
-    r = plan.solve(1e-11, 60, base)
+    log(f"plan of 256 x 10k-bus scenarios: {time.time() - t0:.1f} s")
+    t0 = time.time()
+    r = plan.solve(1e-12, 60, base)
     assert r["converged"].all()
     out = plan.adjoint(1.0, 1.0)
-    nb, nr = net.n_bus, net.n_branch
+    log(f"solve + adjoint of 256 scenarios: {time.time() - t0:.2f} s, iterations <= {int(r['iterations'].max())}")
     rng = np.random.default_rng(11)
-    brs = rng.choice(nr, 300, replace=False)
-    gp, gq, gr, gx, L = reference_gradient(oracle, scen[0], r["v"][0], base, 1.0, 1.0, branches=brs)
-    errs = (rel_err(out["grad_p"][:nb], gp), rel_err(out["grad_q"][:nb], gq),
-            rel_err(out["grad_r"][:nr][brs], gr[brs]), rel_err(out["grad_x"][:nr][brs], gx[brs]))
-    print("10k adjoint vs sparse reference, rel err p q r x:", errs)
-    assert abs(out["loss"][0] - L) <= 1e-11 * abs(L)
-    assert max(errs) < 1e-6
-    # eighth-order central differences of re-solves on scenario 1, at the
-    # parameters with the largest gradients (where a difference quotient can
-    # resolve 1e-6) and at random ones (held to the measured noise floor)
-    from fd_check import GRAD_KEY, assess, fd_gradients, noise_floor
-    g1 = {k: out[k][nb:2 * nb] if k in ("grad_p", "grad_q") else out[k][nr:2 * nr] for k in
-          ("grad_p", "grad_q", "grad_r", "grad_x")}
-    pq_buses = np.nonzero(scen[1].bus_type == PQ)[0]
-    params = [(1, "br_r", int(k)) for k in np.argsort(-np.abs(g1["grad_r"]))[:3]]
-    params += [(1, "br_x", int(k)) for k in np.argsort(-np.abs(g1["grad_x"]))[:3]]
-    params += [(1, "p_w", int(i)) for i in pq_buses[np.argsort(-np.abs(g1["grad_p"][pq_buses]))[:2]]]
-    params += [(1, "p_w", int(i)) for i in rng.choice(pq_buses, 2, replace=False)]
-    params += [(1, "q_var", int(i)) for i in rng.choice(pq_buses, 2, replace=False)]
-    params += [(1, arr, int(k)) for arr in ("br_r", "br_x") for k in rng.choice(nr, 2, replace=False)]
-    g_fd, hs = fd_gradients(oracle, scen, params, 8, 3e-2, base)
-    sigma = noise_floor(oracle, scen[1], base)
-    g_adj = np.array([g1[GRAD_KEY[a]][k] for _, a, k in params])
-    rep = assess(g_adj, g_fd, hs, sigma, 8)
-    print("10k adjoint vs 8th-order FD (h = 3e-2 rel): noise floor of L", sigma,
-          {k: v for k, v in rep.items() if k not in ("ok", "resolvable_mask")})
-    assert rep["ok"].all()
-    assert rep["resolvable_mask"][:6].sum() >= 2
-    assert rep["resolvable_max_rel"] < 1e-6
+    brs = rng.choice(nr, 400, replace=False)
+    for s in (0, 137):
+        t0 = time.time()
+        gp, gq, gr, gx, L = reference_gradient(oracle, scen[s], r["v"][s], base, 1.0, 1.0, branches=brs)
+        sl, sr = slice(s * nb, (s + 1) * nb), slice(s * nr, (s + 1) * nr)
+        errs = (rel_err(out["grad_p"][sl], gp), rel_err(out["grad_q"][sl], gq),
+                rel_err(out["grad_r"][sr][brs], gr[brs]), rel_err(out["grad_x"][sr][brs], gx[brs]))
+        log(f"scenario {s}: vs sparse reference rel err p q r x {errs} ({time.time() - t0:.1f} s)")
+        assert abs(out["loss"][s] - L) <= 1e-11 * abs(L)
+        assert max(errs) < 1e-6
+    s = 137
+    g = {k: out[k][s * nb:(s + 1) * nb] if k in ("grad_p", "grad_q") else out[k][s * nr:(s + 1) * nr]
+         for k in ("grad_p", "grad_q", "grad_r", "grad_x")}
+    pq_buses = np.nonzero(scen[s].bus_type == PQ)[0]
+    top = [(s, "br_r", int(k)) for k in np.argsort(-np.abs(g["grad_r"]))[:3]]
+    top += [(s, "br_x", int(k)) for k in np.argsort(-np.abs(g["grad_x"]))[:3]]
+    params = top + [(s, "p_w", int(i)) for i in pq_buses[np.argsort(-np.abs(g["grad_p"][pq_buses]))[:2]]]
+    params += [(s, "q_var", int(i)) for i in pq_buses[np.argsort(-np.abs(g["grad_q"][pq_buses]))[:1]]]
+    params += [(s, arr, int(i)) for arr in ("p_w", "q_var") for i in rng.choice(pq_buses, 2, replace=False)]
+    params += [(s, arr, int(k)) for arr in ("br_r", "br_x") for k in rng.choice(nr, 2, replace=False)]
+    sigma = noise_floor(oracle, scen[s], base, seed=s)
+    for order, h_rel, plist in ((8, 3e-2, params), (2, 1e-6, top)):
+        t0 = time.time()
+        g_fd, hs = fd_gradients(oracle, scen, plist, order, h_rel, base)
+        g_adj = np.array([g[GRAD_KEY[a]][k] for _, a, k in plist])
+        rep = assess(g_adj, g_fd, hs, sigma, order)
+        log(f"FD order {order}, h = {h_rel:g} rel, {len(plist)} parameters ({time.time() - t0:.1f} s): noise floor "
+            f"of L {sigma:.2e}, {({k: v for k, v in rep.items() if k not in ('ok', 'resolvable_mask')})}")
+        assert rep["ok"].all()
+        if order == 8:
+            assert rep["resolvable_mask"][:6].sum() >= 2
+            assert rep["resolvable_max_rel"] < 1e-6
 
 
 def test_adjoint_batch_equals_single(device):
@@ -276,3 +300,61 @@ def test_torch_autograd(device):
     ref2 = plan.adjoint(0.0, 0.0, gvm.cpu().numpy(), gva.cpu().numpy())
     assert np.allclose(q2.grad.cpu().numpy(), ref2["grad_q"], rtol=1e-12, atol=0)
     assert np.allclose(x2.grad.cpu().numpy(), ref2["grad_x"], rtol=1e-12, atol=0)
+
+
+def test_torch_autograd_interleaved_forwards(device):
+    """Two forwards on one plan with different loads, backward of the first
+    after the second: each output gets the gradients of its own solve (the
+    wrapper re-solves the first inputs), equal to a fresh plan's."""
+    import torch
+    from gap_amd.autograd import PowerFlow, SolveOptions
+    base = 100e6
+    net = gridgen.small_mesh(150, seed=4, n_extra=20)
+    plan = capi.Plan([net])
+    opts = SolveOptions(1e-11, 60, base)
+    t = lambda a: torch.tensor(a, dtype=torch.float64, device="cuda", requires_grad=True)
+    ins1 = [t(net.p_w), t(net.q_var), t(net.br_r), t(net.br_x)]
+    ins2 = [t(net.p_w * 1.25), t(net.q_var * 0.8), t(net.br_r), t(net.br_x * 1.1)]
+    vm1, va1 = PowerFlow.apply(*ins1, plan, opts)
+    vm2, va2 = PowerFlow.apply(*ins2, plan, opts)
+    assert not torch.equal(vm1, vm2)
+    g = torch.linspace(-1.0, 1.0, net.n_bus, dtype=torch.float64, device="cuda")
+    (vm1 * g).sum().backward()
+    fresh = capi.Plan([net])
+    fresh.solve(1e-11, 60, base)
+    ref = fresh.adjoint(0.0, 0.0, g.cpu().numpy(), None)
+    assert np.allclose(ins1[0].grad.cpu().numpy(), ref["grad_p"], rtol=1e-10, atol=0)
+    assert np.allclose(ins1[3].grad.cpu().numpy(), ref["grad_x"], rtol=1e-10, atol=0)
+    (vm2 * g).sum().backward()
+    n2 = net.copy()
+    n2.p_w, n2.q_var, n2.br_x = net.p_w * 1.25, net.q_var * 0.8, net.br_x * 1.1
+    fresh2 = capi.Plan([n2])
+    fresh2.solve(1e-11, 60, base)
+    ref2 = fresh2.adjoint(0.0, 0.0, g.cpu().numpy(), None)
+    assert np.allclose(ins2[0].grad.cpu().numpy(), ref2["grad_p"], rtol=1e-10, atol=0)
+
+
+def test_adjoint_and_flows_refuse_stale_state(device):
+    """The adjoint needs a converged solve of the current inputs, the branch
+    flows a solve of the current branch data."""
+    net = gridgen.small_mesh(60, seed=8)
+    plan = capi.Plan([net])
+    plan.solve(1e-10, 60, 100e6)
+    plan.adjoint(1.0, 1.0)
+    plan.branch_flows()
+    plan.set_injections(0, net.p_w * 1.1, net.q_var)
+    with pytest.raises(capi.GapHipError) as e:
+        plan.adjoint(1.0, 1.0)
+    assert e.value.code == capi.E_STATE
+    plan.branch_flows()  # V and Y unchanged: still the last solution's flows
+    plan.set_branch_rx(0, net.br_r * 1.1, net.br_x)
+    with pytest.raises(capi.GapHipError) as e:
+        plan.branch_flows()
+    assert e.value.code == capi.E_STATE
+    plan.solve(1e-10, 60, 100e6)
+    plan.adjoint(1.0, 1.0)
+    plan.branch_flows()
+    plan.solve(1e-10, 2, 100e6)  # stops at the iteration cap
+    with pytest.raises(capi.GapHipError) as e:
+        plan.adjoint(1.0, 1.0)
+    assert e.value.code == capi.E_STATE
