@@ -92,6 +92,8 @@ SIGNATURES = {
     "gaphip_mismatches": (C.c_int, [C.c_void_p, C.c_int, _d, C.c_double, _d, C.c_int, _i]),
     "gaphip_power_mismatches": (C.c_int, [C.POINTER(GaphipNet), _d, C.c_int, _i, _i, _d, C.c_int, C.c_double, _d,
                                           C.c_int, _i]),
+    "gaphip_power_mismatches_noalloc": (C.c_int, [C.POINTER(GaphipNet), _d, C.c_int, _i, _i, _d, C.c_int, C.c_double,
+                                                  _d, C.c_int, _i]),
     "gaphip_adjoint": (C.c_int, [C.c_void_p, C.c_double, C.c_double, _d, _d, _d, _d, _d, _d, _d]),
     "gaphip_branch_flows": (C.c_int, [C.c_void_p, _d]),
     "gaphip_plan_update_branches": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _i, _d, _d]),

@@ -22,6 +22,7 @@
 #include "../../include/gap_hip.h"
 #include "kernels.hpp"
 #include "fused.hpp"
+#include "evaluate.hpp"
 #include "symbolic.hpp"
 
 namespace gaphip {
@@ -2029,6 +2030,108 @@ int gaphip_kernel_count(void) { return K_COUNT; }
 // ------------------------------------------------------------ inspection
 namespace gaphip {
 
+// Device memory of gaphip_power_mismatches_noalloc: one static array per GPU,
+// sized for the largest evaluation it accepts (the EvalLayout of its limits,
+// about 8.4 MB), held until the process ends.  Host code reaches it only
+// through the address hipGetSymbolAddress returns on each device (the array's
+// name is a host-side stand-in); that address is looked up once per device, on
+// the entry point's first call there, and kept.  Calls on one device are
+// serialised by g_scratch_call; every call overwrites the previous contents.
+constexpr size_t kNoAllocScratchBytes =
+    EvalLayout(GAPHIP_NOALLOC_MAX_BUS, GAPHIP_NOALLOC_MAX_BUS, GAPHIP_NOALLOC_MAX_NNZ, GAPHIP_NOALLOC_MAX_APP).total;
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kScratchDevices] = {};
+std::mutex g_scratch_call[kScratchDevices];
+
+}  // namespace gaphip
+
+static __device__ __align__(256) unsigned char scratch[gaphip::kNoAllocScratchBytes];
+
+namespace gaphip {
+
+// Device address of `scratch` on device `dev` (current), looked up once.
+unsigned char* scratch_address(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        hipError_t const e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(GAPHIP_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_dev[dev];
+}
+
+// Argument checks of the mismatch entry points (no copy of the network: the
+// no-allocation variant must not allocate); sets *m_out to the row count.
+void check_eval_args(gaphip_net const* net, double const* v_ri, int y_rows, int const* rp, int const* ci,
+                     double const* y_ri, int y_nnz, double base, double const* f, int cap, int* m_out) {
+    if (!net || !m_out) throw Error(GAPHIP_E_ARG, "null argument");
+    if (net->n_bus <= 0) throw Error(GAPHIP_E_ARG, "network has no buses");
+    if (net->n_app < 0) throw Error(GAPHIP_E_ARG, "negative appliance count");
+    if (!v_ri || !net->bus_id || !net->bus_type || !net->p_w || !net->q_var)
+        throw Error(GAPHIP_E_ARG, "missing bus array or voltages");
+    if (net->n_app > 0 && (!net->app_node || !net->app_status || !net->app_type || !net->app_p || !net->app_q))
+        throw Error(GAPHIP_E_ARG, "missing appliance array");
+    if (!(base > 0.0)) throw Error(GAPHIP_E_ARG, "base_power must be > 0");
+    if (y_nnz < 0 || y_rows < 0) throw Error(GAPHIP_E_ARG, "negative admittance size");
+    if (y_nnz > 0 && ((y_rows > 0 && !rp) || !ci || !y_ri)) throw Error(GAPHIP_E_ARG, "missing admittance array");
+    int m = 0;
+    for (int i = 0; i < net->n_bus; ++i) m += net->bus_type[i] == kSlack ? 0 : (net->bus_type[i] == kPQ ? 2 : 1);
+    *m_out = m;
+    if (m > cap) throw Error(GAPHIP_E_CAPACITY, "mismatch capacity too small");
+    if (m > 0 && !f) throw Error(GAPHIP_E_ARG, "null output");
+}
+
+// Body shared by the two mismatch entry points: stages the inputs into the
+// device region laid out by L, runs the row scan and the per-bus evaluation on
+// stream s, copies F out.  rows = min(y_rows, n_bus).
+void run_power_mismatches(gaphip_net const& net, double const* v_ri, int rows, int const* rp, int const* ci,
+                          double const* y_ri, int y_nnz, double base, double* f, unsigned char* region,
+                          EvalLayout const& L, hipStream_t s) {
+    int const n = net.n_bus, na = net.n_app;
+    auto at = [&](size_t off) { return region + off; };
+    auto put = [&](size_t off, void const* src, size_t bytes) {
+        if (bytes) HIPCHK(hipMemcpyAsync(at(off), src, bytes, hipMemcpyHostToDevice, s));
+    };
+    bool const have_y = y_nnz > 0 && rows > 0;
+    if (have_y) {
+        put(L.rp, rp, sizeof(int) * (static_cast<size_t>(rows) + 1));
+        put(L.ci, ci, sizeof(int) * static_cast<size_t>(y_nnz));
+        put(L.y, y_ri, sizeof(double2) * static_cast<size_t>(y_nnz));
+    }
+    put(L.v, v_ri, sizeof(double2) * n);
+    put(L.bus_id, net.bus_id, sizeof(int) * n);
+    put(L.bus_type, net.bus_type, sizeof(int) * n);
+    put(L.p_w, net.p_w, sizeof(double) * n);
+    put(L.q_var, net.q_var, sizeof(double) * n);
+    if (na > 0) {
+        put(L.app_node, net.app_node, sizeof(int) * na);
+        put(L.app_status, net.app_status, sizeof(int) * na);
+        put(L.app_type, net.app_type, sizeof(int) * na);
+        put(L.app_p, net.app_p, sizeof(double) * na);
+        put(L.app_q, net.app_q, sizeof(double) * na);
+    }
+    auto I = [&](size_t off) { return reinterpret_cast<int*>(at(off)); };
+    auto D = [&](size_t off) { return reinterpret_cast<double*>(at(off)); };
+    EvalArgs a{n, have_y ? rows : 0, have_y ? y_nnz : 0, na, base,
+               I(L.rp), I(L.ci), reinterpret_cast<double2 const*>(at(L.y)), reinterpret_cast<double2 const*>(at(L.v)),
+               I(L.bus_id), I(L.bus_type), D(L.p_w), D(L.q_var),
+               I(L.app_node), I(L.app_status), I(L.app_type), D(L.app_p), D(L.app_q),
+               I(L.peq), I(L.qeq), D(L.f), I(L.err)};
+    hipLaunchKernelGGL(k_eval_rows, dim3(1), dim3(kEvalScanThreads), 0, s, a);
+    hipLaunchKernelGGL(k_eval_mismatch, dim3(blocks_for(n, 256)), dim3(256), 0, s, a);
+    HIPCHK(hipGetLastError());
+    int err = 0;
+    HIPCHK(hipMemcpyAsync(&err, at(L.err), sizeof(int), hipMemcpyDeviceToHost, s));
+    int m = 0;
+    for (int i = 0; i < n; ++i) m += net.bus_type[i] == kSlack ? 0 : (net.bus_type[i] == kPQ ? 2 : 1);
+    if (m) HIPCHK(hipMemcpyAsync(f, at(L.f), sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (err) throw Error(GAPHIP_E_ARG, "negative admittance row pointer or column index");
+}
+
 // Evaluates Y, S, F and (optionally) J of one instance at the given voltages.
 void evaluate_instance(gaphip_plan& P, int inst, double const* v_ri, double base, bool want_j) {
     hipStream_t s = P.stream;
@@ -2116,99 +2219,44 @@ int gaphip_power_mismatches(const gaphip_net* net, const double* v_ri, int y_row
                             const int* y_col_idx, const double* y_ri, int y_nnz, double base_power, double* f,
                             int cap, int* m_out) {
     GAPHIP_GUARD({
-        if (!net || !m_out || (net->n_bus > 0 && !v_ri)) throw Error(GAPHIP_E_ARG, "null argument");
-        if (!(base_power > 0.0)) throw Error(GAPHIP_E_ARG, "base_power must be > 0");
-        if (y_nnz < 0 || y_rows < 0) throw Error(GAPHIP_E_ARG, "negative admittance size");
-        if (y_nnz > 0 && (!y_row_ptr || !y_col_idx || !y_ri)) throw Error(GAPHIP_E_ARG, "missing admittance array");
+        check_eval_args(net, v_ri, y_rows, y_row_ptr, y_col_idx, y_ri, y_nnz, base_power, f, cap, m_out);
         int dev_count = 0;
         if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
             throw Error(GAPHIP_E_DEVICE, "no HIP device visible");
-        Net const N = copy_net(*net);
-        int const n = N.n_bus();
-        // equation rows: P for every non-slack bus, then Q if PQ, in bus order
-        // (cpu_newton_raphson.cpp:269-289)
-        std::vector<int> peq(n, -1), qeq(n, -1);
-        int m = 0;
-        for (int i = 0; i < n; ++i) {
-            if (N.bus_type[i] == kSlack) continue;
-            peq[i] = m++;
-            if (N.bus_type[i] == kPQ) qeq[i] = m++;
-        }
-        *m_out = m;
-        if (m > cap) throw Error(GAPHIP_E_CAPACITY, "mismatch capacity too small");
-        if (m > 0 && !f) throw Error(GAPHIP_E_ARG, "null output");
-        // The admittance matrix as the reference walks it (:296-337): rows past
-        // y_rows give zero current, a row stops at the end of the arrays and
-        // skips columns >= n_bus; y_nnz = 0 is the empty matrix (zero power).
-        std::vector<int> rp(n + 1, 0), ci;
-        std::vector<double2> yv;
-        for (int i = 0; i < n && y_nnz > 0; ++i) {
-            if (i < y_rows) {
-                for (int q = y_row_ptr[i]; q < y_row_ptr[i + 1] && q < y_nnz; ++q) {
-                    if (q < 0) throw Error(GAPHIP_E_ARG, "admittance row pointer out of range");
-                    int const k = y_col_idx[q];
-                    if (k < 0) throw Error(GAPHIP_E_ARG, "negative admittance column");
-                    if (k >= n) continue;
-                    ci.push_back(k);
-                    yv.push_back(make_double2(y_ri[2 * q], y_ri[2 * q + 1]));
-                }
-            }
-            rp[i + 1] = static_cast<int>(ci.size());
-        }
-        if (y_nnz <= 0) std::fill(rp.begin(), rp.end(), 0);
-        // specified power (:346-377): the bus's own injection when non-zero,
-        // plus every in-service SOURCE / LOADGEN appliance whose node equals the
-        // bus id, in appliance order, over base_power
-        std::vector<double2> sp(n);
-        for (int i = 0; i < n; ++i) {
-            double pr = 0.0, pi = 0.0;
-            if (N.p_w[i] != 0.0 || N.q_var[i] != 0.0) {
-                pr = N.p_w[i];
-                pi = N.q_var[i];
-            }
-            for (int a = 0; a < N.n_app(); ++a)
-                if (N.app_node[a] == N.bus_id[i] && N.app_status[a] == 1 &&
-                    (N.app_type[a] == kSource || N.app_type[a] == kLoadgen)) {
-                    pr += N.app_p[a];
-                    pi += N.app_q[a];
-                }
-            sp[i] = make_double2(pr / base_power, pi / base_power);
-        }
+        int const rows = std::min(y_rows, net->n_bus);
+        EvalLayout const L(net->n_bus, rows, y_nnz, net->n_app);
+        DBuf<unsigned char> region;
+        region.alloc(L.total);
         hipStream_t s;
         HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         struct StreamGuard {
             hipStream_t s;
             ~StreamGuard() { (void)hipStreamDestroy(s); }
         } guard{s};
-        std::vector<double2> v(n);
-        for (int i = 0; i < n; ++i) v[i] = make_double2(v_ri[2 * i], v_ri[2 * i + 1]);
-        if (ci.empty()) {
-            ci.push_back(0);  // never read (every row is empty); keeps the buffers non-null
-            yv.push_back(make_double2(0.0, 0.0));
-        }
-        DBuf<int> d_rp, d_ci, d_peq, d_qeq, d_inst, d_done;
-        DBuf<double2> d_y, d_v, d_sp, d_s, d_vpol;
-        DBuf<double> d_f;
-        DBuf<unsigned long long> d_max;
-        d_rp.upload(rp, s);
-        d_ci.upload(ci, s);
-        d_y.upload(yv, s);
-        d_v.upload(v, s);
-        d_sp.upload(sp, s);
-        d_peq.upload(peq, s);
-        d_qeq.upload(qeq, s);
-        d_inst.upload(std::vector<int>(n, 0), s);
-        d_done.upload(std::vector<int>{0}, s);
-        d_max.upload(std::vector<unsigned long long>{0}, s);
-        d_s.alloc(n);
-        d_vpol.alloc(n);
-        d_f.alloc(std::max(m, 1));
-        MisArgs ma{n, d_rp.p, d_ci.p, d_y.p, d_v.p, d_sp.p, d_peq.p, d_qeq.p, d_inst.p, d_done.p, d_s.p, d_vpol.p,
-                   d_f.p, d_max.p};
-        if (n > 0) hipLaunchKernelGGL(k_mismatch, dim3(blocks_for(n, 256)), dim3(256), 0, s, ma);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(s));
-        if (m) HIPCHK(hipMemcpy(f, d_f.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+        run_power_mismatches(*net, v_ri, rows, y_row_ptr, y_col_idx, y_ri, y_nnz, base_power, f, region.p, L, s);
+    })
+}
+
+int gaphip_power_mismatches_noalloc(const gaphip_net* net, const double* v_ri, int y_rows, const int* y_row_ptr,
+                                    const int* y_col_idx, const double* y_ri, int y_nnz, double base_power, double* f,
+                                    int cap, int* m_out) {
+    GAPHIP_GUARD({
+        check_eval_args(net, v_ri, y_rows, y_row_ptr, y_col_idx, y_ri, y_nnz, base_power, f, cap, m_out);
+        if (net->n_bus > GAPHIP_NOALLOC_MAX_BUS || y_nnz > GAPHIP_NOALLOC_MAX_NNZ || net->n_app > GAPHIP_NOALLOC_MAX_APP)
+            throw Error(GAPHIP_E_CAPACITY,
+                        "gaphip_power_mismatches_noalloc: network above the limits of its static device array "
+                        "(at most 32768 buses, 262144 admittance entries, 32768 appliances)");
+        int dev_count = 0, dev = 0;
+        if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+            throw Error(GAPHIP_E_DEVICE, "no HIP device visible");
+        HIPCHK(hipGetDevice(&dev));
+        if (dev < 0 || dev >= kScratchDevices) throw Error(GAPHIP_E_DEVICE, "device ordinal above 63");
+        unsigned char* const region = scratch_address(dev);
+        int const rows = std::min(y_rows, net->n_bus);
+        EvalLayout const L(net->n_bus, rows, y_nnz, net->n_app);
+        if (L.total > kNoAllocScratchBytes) throw Error(GAPHIP_E_CAPACITY, "evaluation larger than the static array");
+        std::lock_guard<std::mutex> lk(g_scratch_call[dev]);
+        run_power_mismatches(*net, v_ri, rows, y_row_ptr, y_col_idx, y_ri, y_nnz, base_power, f, region, L, nullptr);
     })
 }
 

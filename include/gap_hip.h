@@ -217,6 +217,24 @@ int gaphip_power_mismatches(const gaphip_net* net, const double* v_ri, int y_row
                             const int* y_col_idx, const double* y_ri, int y_nnz, double base_power, double* f,
                             int cap, int* m_out);
 
+/* The same evaluation with no device allocation: no counterpart in the
+ * reference, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as gaphip_power_mismatches (its kernels and
+ * arithmetic, bit for bit); every byte of device memory it touches lies in one
+ * static array per GPU, sized for the limits below (about 8.4 MB, held until
+ * the process ends).  Larger inputs return GAPHIP_E_CAPACITY.  It calls no
+ * hipMalloc / hipFree and creates no stream: copies and kernels run on the
+ * null stream and the call returns when F is on the host.  Calls on one GPU
+ * share its array: this library serialises them with a lock, and each call
+ * overwrites the previous call's contents. */
+#define GAPHIP_NOALLOC_MAX_BUS 32768
+#define GAPHIP_NOALLOC_MAX_NNZ 262144
+#define GAPHIP_NOALLOC_MAX_APP 32768
+int gaphip_power_mismatches_noalloc(const gaphip_net* net, const double* v_ri, int y_rows, const int* y_row_ptr,
+                                    const int* y_col_idx, const double* y_ri, int y_nnz, double base_power,
+                                    double* f, int cap, int* m_out);
+
 /* Adjoint sensitivities at the converged state of the last solve (FP64):
  * solves J^T lambda = dL/dx with the factors of J at the solution and returns
  * dL/d(bus p_w, bus q_var) [per W / VAr] and dL/d(branch r1, x1) [per ohm] for
