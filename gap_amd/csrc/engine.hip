@@ -10,6 +10,9 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <queue>
+#include <limits>
+#include <complex>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -2308,7 +2311,10 @@ int gaphip_ybus_build(const gaphip_net* net, int* row_ptr, int* col_idx, double*
 struct gaphip_lu {
     int n = 0;
     bool factorized = false;
-    std::vector<int> pat_rp, pat_ci;  // pattern the symbolic plan was built for
+    std::vector<int> pat_rp, pat_ci;  // pattern the symbolic plan was built for (permuted, duplicate-free)
+    std::vector<int> row_of;          // row j of the factored matrix is row row_of[j] of the caller's
+    std::vector<std::complex<double>> row_scale;  // per caller row: scaling (magnitude and phase)
+    std::vector<double> col_scale;    // per column
     std::vector<int> uniq_of;         // CSR entry -> unique (i, j) index
     int n_uniq = 0;
     gaphip::LUPlan plan;
@@ -2322,6 +2328,118 @@ struct gaphip_lu {
 };
 
 namespace gaphip {
+
+// Maximum-product transversal with scaling (the approach of Duff and Koster's
+// MC64): for every column j a distinct row i = row_of[j] such that the product
+// of the |a_ij| is maximal, as a minimum-cost perfect matching with costs
+// c_ij = log(max_k |a_kj|) - log |a_ij| >= 0, found by one shortest augmenting
+// path per column (Dijkstra on the reduced costs c_ij - u_j - v_i >= 0, with
+// the dual variables updated after each path).  At the optimum every matched
+// entry has reduced cost 0 and every other a non-negative one, so the row
+// scales exp(v_i) and column scales exp(u_j) / max_k |a_kj| make the matched
+// entries 1 in magnitude and every other entry at most 1.  Zero entries are
+// not edges.  Returns false when some column cannot be matched (the matrix is
+// structurally singular).  rp / ci / mag: duplicate-free CSR of |A|.
+bool max_product_matching(int n, std::vector<int> const& rp, std::vector<int> const& ci,
+                          std::vector<double> const& mag, std::vector<int>& row_of, std::vector<double>& row_scale,
+                          std::vector<double>& col_scale) {
+    // columns as lists of (row, cost)
+    std::vector<double> amax(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int q = rp[i]; q < rp[i + 1]; ++q) amax[ci[q]] = std::max(amax[ci[q]], mag[q]);
+    std::vector<int> cp(n + 1, 0);
+    for (int i = 0; i < n; ++i)
+        for (int q = rp[i]; q < rp[i + 1]; ++q)
+            if (mag[q] > 0.0) cp[ci[q] + 1]++;
+    for (int j = 0; j < n; ++j) cp[j + 1] += cp[j];
+    std::vector<int> cr(cp[n]), fill(cp.begin(), cp.end() - 1);
+    std::vector<double> cc(cp[n]);
+    for (int i = 0; i < n; ++i)
+        for (int q = rp[i]; q < rp[i + 1]; ++q) {
+            int const j = ci[q];
+            if (!(mag[q] > 0.0)) continue;
+            cr[fill[j]] = i;
+            cc[fill[j]++] = std::log(amax[j]) - std::log(mag[q]);
+        }
+    for (int j = 0; j < n; ++j)
+        if (cp[j] == cp[j + 1]) return false;
+    std::vector<double> u(n, 0.0), v(n, 0.0);  // column and row duals (all costs are >= 0)
+    std::vector<int> mate_row(n, -1), mate_col(n, -1);
+    for (int j = 0; j < n; ++j)  // greedy start on zero-cost entries
+        for (int q = cp[j]; q < cp[j + 1]; ++q)
+            if (cc[q] == 0.0 && mate_row[cr[q]] < 0) {
+                mate_row[cr[q]] = j;
+                mate_col[j] = cr[q];
+                break;
+            }
+    double const inf = std::numeric_limits<double>::infinity();
+    std::vector<double> dist(n, inf);
+    std::vector<int> pred(n, -1);
+    std::vector<char> done(n, 0);
+    std::vector<int> touched, final_rows;
+    using Item = std::pair<double, int>;
+    for (int j0 = 0; j0 < n; ++j0) {
+        if (mate_col[j0] >= 0) continue;
+        std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+        auto relax = [&](int j, double base) {
+            for (int q = cp[j]; q < cp[j + 1]; ++q) {
+                int const k = cr[q];
+                if (done[k]) continue;
+                double const nd = base + std::max(0.0, cc[q] - u[j] - v[k]);
+                if (nd < dist[k]) {
+                    if (dist[k] == inf) touched.push_back(k);
+                    dist[k] = nd;
+                    pred[k] = j;
+                    heap.push({nd, k});
+                }
+            }
+        };
+        relax(j0, 0.0);
+        int sink = -1;
+        double dstar = 0.0;
+        while (!heap.empty()) {
+            auto const [d, i] = heap.top();
+            heap.pop();
+            if (done[i] || d > dist[i]) continue;
+            done[i] = 1;
+            final_rows.push_back(i);
+            if (mate_row[i] < 0) {
+                sink = i;
+                dstar = d;
+                break;
+            }
+            relax(mate_row[i], d);
+        }
+        if (sink < 0) return false;
+        u[j0] += dstar;
+        for (int i : final_rows) {
+            double const delta = dstar - dist[i];
+            if (i != sink) u[mate_row[i]] += delta;
+            v[i] -= delta;
+        }
+        for (int i = sink;;) {
+            int const j = pred[i];
+            int const prev = mate_col[j];
+            mate_col[j] = i;
+            mate_row[i] = j;
+            if (j == j0) break;
+            i = prev;
+        }
+        for (int k : touched) {
+            dist[k] = inf;
+            pred[k] = -1;
+            done[k] = 0;
+        }
+        touched.clear();
+        final_rows.clear();
+    }
+    row_of = mate_col;
+    row_scale.resize(n);
+    col_scale.resize(n);
+    for (int i = 0; i < n; ++i) row_scale[i] = std::exp(v[i]);
+    for (int j = 0; j < n; ++j) col_scale[j] = std::exp(u[j]) / amax[j];
+    return true;
+}
 
 // Complex matrix as a real system of twice the size: node i carries (re, im),
 // entry a+ib of (i, j) becomes [[a, -b], [b, a]] on unknowns (2i, 2i+1) x (2j, 2j+1).
@@ -2417,21 +2535,71 @@ int gaphip_lu_factorize(gaphip_lu* L, int n_rows, int n_cols, const int* rp, con
             if (rp[i + 1] < rp[i]) throw Error(GAPHIP_E_ARG, "row_ptr not monotone");
         for (int q = 0; q < nnz; ++q)
             if (ci[q] < 0 || ci[q] >= n) throw Error(GAPHIP_E_ARG, "column index out of range");
-        bool same = L->n == n && static_cast<int>(L->pat_ci.size()) == nnz &&
-                    std::equal(rp, rp + n + 1, L->pat_rp.begin()) && std::equal(ci, ci + nnz, L->pat_ci.begin());
-        if (!same) lu_symbolic(*L, n, rp, ci);
-        // expanded real values in unique-entry order, duplicates summed
-        std::vector<double> ur(L->n_uniq, 0.0), ui(L->n_uniq, 0.0);
-        for (int q = 0; q < nnz; ++q) {
-            ur[L->uniq_of[q]] += v_ri[2 * q];
-            ui[L->uniq_of[q]] += v_ri[2 * q + 1];
+        // duplicates summed onto their first occurrence in the row
+        std::vector<int> urp(n + 1, 0), uci;
+        std::vector<std::complex<double>> uval;
+        {
+            std::vector<int> at(n, -1);
+            for (int i = 0; i < n; ++i) {
+                for (int q = rp[i]; q < rp[i + 1]; ++q) {
+                    int const j = ci[q];
+                    std::complex<double> const a(v_ri[2 * q], v_ri[2 * q + 1]);
+                    if (at[j] >= urp[i]) {
+                        uval[at[j]] += a;
+                        continue;
+                    }
+                    at[j] = static_cast<int>(uci.size());
+                    uci.push_back(j);
+                    uval.push_back(a);
+                }
+                urp[i + 1] = static_cast<int>(uci.size());
+            }
         }
+        // row permutation and scalings that put entries of magnitude 1 on the
+        // diagonal with every other entry at most 1 (the reference pivots by
+        // column maximum instead, cpu_lu_solver.cpp:469-498; the factorisation
+        // below has static pivots)
+        std::vector<double> mag(uci.size());
+        for (size_t q = 0; q < uci.size(); ++q) mag[q] = std::abs(uval[q]);
+        std::vector<double> rsc, csc;
+        if (!max_product_matching(n, urp, uci, mag, L->row_of, rsc, csc))
+            throw Error(GAPHIP_E_SINGULAR, "matrix is structurally singular (no row for some column)");
+        // permuted, scaled matrix: row j = scaled row row_of[j]; each row also
+        // takes the phase that makes its diagonal entry real and positive
+        L->row_scale.assign(n, 0.0);
+        L->col_scale = csc;
+        std::vector<int> prp(n + 1, 0), pci;
+        std::vector<double> pv;
+        pci.reserve(uci.size());
+        pv.reserve(2 * uci.size());
+        for (int j = 0; j < n; ++j) {
+            int const i = L->row_of[j];
+            std::complex<double> diag(0.0);
+            for (int q = urp[i]; q < urp[i + 1]; ++q)
+                if (uci[q] == j) diag = uval[q] * (rsc[i] * csc[j]);
+            std::complex<double> const phase = std::abs(diag) > 0.0 ? std::conj(diag) / std::abs(diag) : 1.0;
+            L->row_scale[i] = rsc[i] * phase;
+            for (int q = urp[i]; q < urp[i + 1]; ++q) {
+                std::complex<double> const a = uval[q] * L->row_scale[i] * csc[uci[q]];
+                pci.push_back(uci[q]);
+                pv.push_back(a.real());
+                pv.push_back(a.imag());
+            }
+            prp[j + 1] = static_cast<int>(pci.size());
+        }
+        int const pnz = prp[n];
+        bool same = L->n == n && static_cast<int>(L->pat_ci.size()) == pnz &&
+                    std::equal(prp.begin(), prp.end(), L->pat_rp.begin()) &&
+                    std::equal(pci.begin(), pci.end(), L->pat_ci.begin());
+        if (!same) lu_symbolic(*L, n, prp.data(), pci.data());
+        // expanded real values in unique-entry order (no duplicates remain)
         std::vector<double> ex(4 * static_cast<size_t>(L->n_uniq));
-        for (int u = 0; u < L->n_uniq; ++u) {
-            ex[4 * u + 0] = ur[u];
-            ex[4 * u + 1] = -ui[u];
-            ex[4 * u + 2] = ui[u];
-            ex[4 * u + 3] = ur[u];
+        for (int q = 0; q < pnz; ++q) {
+            int const u = L->uniq_of[q];
+            ex[4 * u + 0] = pv[2 * q];
+            ex[4 * u + 1] = -pv[2 * q + 1];
+            ex[4 * u + 2] = pv[2 * q + 1];
+            ex[4 * u + 3] = pv[2 * q];
         }
         L->vals.upload(ex, L->stream);
         HIPCHK(hipMemsetAsync(L->fail.p, 0, sizeof(int), L->stream));
@@ -2452,13 +2620,25 @@ int gaphip_lu_solve(gaphip_lu* L, const double* rhs_ri, double* x_ri, int n) {
         if (!L->factorized) throw Error(GAPHIP_E_STATE, "Matrix not factorized. Call factorize() first.");
         if (n != L->n) throw Error(GAPHIP_E_ARG, "RHS size does not match matrix dimension");
         if (!rhs_ri || !x_ri) throw Error(GAPHIP_E_ARG, "null vector");
-        HIPCHK(hipMemcpyAsync(L->rhs.p, rhs_ri, sizeof(double) * 2 * n, hipMemcpyHostToDevice, L->stream));
+        // A x = b  <=>  (P Dr A Dc) y = P Dr b,  x = Dc y
+        std::vector<double> b(2 * static_cast<size_t>(n));
+        for (int j = 0; j < n; ++j) {
+            int const i = L->row_of[j];
+            std::complex<double> const v = std::complex<double>(rhs_ri[2 * i], rhs_ri[2 * i + 1]) * L->row_scale[i];
+            b[2 * j] = v.real();
+            b[2 * j + 1] = v.imag();
+        }
+        HIPCHK(hipMemcpyAsync(L->rhs.p, b.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, L->stream));
         FrontArgs fa = L->fr.args(nullptr, L->fail.p);
         FrontRunner<double>::forward(L->fr, fa, L->front.p, L->vec.p, L->rhs.p, 1.0, L->stream, 0, PlainHook{});
         FrontRunner<double>::backward(L->fr, fa, L->front.p, L->vec.p, L->sol.p, L->stream, 0, PlainHook{});
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(x_ri, L->sol.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, L->stream));
+        HIPCHK(hipMemcpyAsync(b.data(), L->sol.p, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, L->stream));
         HIPCHK(hipStreamSynchronize(L->stream));
+        for (int c = 0; c < n; ++c) {
+            x_ri[2 * c] = b[2 * c] * L->col_scale[c];
+            x_ri[2 * c + 1] = b[2 * c + 1] * L->col_scale[c];
+        }
     })
 }
 

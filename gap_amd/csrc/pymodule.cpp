@@ -243,6 +243,22 @@ PYBIND11_MODULE(gap_solver, m) {
     py::class_<hip::HipAdmittanceMatrix, admittance::IAdmittanceMatrix, std::shared_ptr<hip::HipAdmittanceMatrix>>(
         m, "HipAdmittanceMatrix")
         .def(py::init<>());
+    py::class_<solver::ILUSolver, std::shared_ptr<solver::ILUSolver>>(m, "ILUSolver")
+        .def("factorize", &solver::ILUSolver::factorize, py::arg("matrix"))
+        .def("solve", &solver::ILUSolver::solve, py::arg("rhs"))
+        .def("update_factorization", &solver::ILUSolver::update_factorization, py::arg("matrix"))
+        .def("get_backend_type", &solver::ILUSolver::get_backend_type)
+        .def("is_factorized", &solver::ILUSolver::is_factorized);
+    py::class_<hip::HipLUSolver, solver::ILUSolver, std::shared_ptr<hip::HipLUSolver>>(m, "HipLUSolver")
+        .def(py::init<>());
+    m.def(
+        "create_lu_solver",
+        [](BackendType backend) -> std::shared_ptr<solver::ILUSolver> {
+            if (backend == BackendType::CPU)
+                throw std::runtime_error("Requested backend 'CPU' is not available (this build has GPU_HIP)");
+            return std::make_shared<hip::HipLUSolver>();
+        },
+        py::arg("backend"), "LU solver of the backend (the reference factory's create_lu_solver).");
     py::class_<solver::IPowerFlowSolver, std::shared_ptr<solver::IPowerFlowSolver>>(m, "IPowerFlowSolver")
         .def("solve_power_flow", &solver::IPowerFlowSolver::solve_power_flow, py::arg("network_data"),
              py::arg("admittance_matrix"), py::arg("config"))

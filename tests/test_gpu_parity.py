@@ -323,3 +323,80 @@ def test_pybind_simple_api(device):
         gs.solve_simple_power_flow(bus, br, backend="cpu")
     with pytest.raises(RuntimeError, match="at least 7"):
         gs.solve_simple_power_flow([[0, 1e4, 2]], br, backend="hip")
+
+
+def _csr(a):
+    rows, cols = np.nonzero(a)
+    rp = np.searchsorted(rows, np.arange(a.shape[0] + 1)).astype(np.int32)
+    return rp, cols.astype(np.int32), a[rows, cols]
+
+
+def _sparse_matrix(a):
+    import gap_amd.gap_solver as gs
+    rp, ci, val = _csr(a)
+    m = gs.SparseMatrix()
+    m.row_ptr, m.col_idx, m.values = rp.tolist(), ci.tolist(), [complex(x) for x in val]
+    m.num_rows = m.num_cols = a.shape[0]
+    m.nnz = len(ci)
+    return m
+
+
+def _non_dominant(rng, n, complex_values=True):
+    """Random sparse matrix with no diagonal dominance: entries U[-1, 1] (complex
+    or real) at density ~10%, plus one entry of magnitude 0.1-1 per row at a
+    random column (so it is structurally non-singular, with most diagonal
+    entries zero)."""
+    a = rng.uniform(-1, 1, (n, n)) * (rng.uniform(0, 1, (n, n)) < 0.1)
+    if complex_values:
+        a = a + 1j * rng.uniform(-1, 1, (n, n)) * (a != 0)
+    p = rng.permutation(n)
+    a = a.astype(complex)
+    a[np.arange(n), p] += rng.uniform(0.1, 1.0, n) * (np.exp(1j * rng.uniform(0, 6.3, n)) if complex_values else 1)
+    return a
+
+
+def test_lu_row_exchanges(device):
+    """Matrices the reference LU solves only with row exchanges
+    (cpu_lu_solver.cpp:469-498): the oracle's zero-diagonal 3x3 and random
+    non-dominant sparse matrices (real, as the Newton Jacobian, and complex),
+    through the C ABI and through the ILUSolver plugin class, against LAPACK."""
+    import gap_amd.gap_solver as gs
+    rng = np.random.default_rng(17)
+    cases = [np.array([[0, 2, 0], [1, 0, 0], [0, 1, 3]], complex)]
+    for n, cplx in ((6, False), (40, True), (150, False), (400, True), (900, False)):
+        while True:
+            a = _non_dominant(rng, n, cplx)
+            if np.linalg.cond(a) < 1e7:
+                break
+        cases.append(a)
+    lu = capi.LU()
+    plug = gs.create_lu_solver(gs.BackendType.GPU_HIP)
+    assert isinstance(plug, gs.HipLUSolver) and plug.get_backend_type() == gs.BackendType.GPU_HIP
+    for a in cases:
+        n = a.shape[0]
+        assert np.count_nonzero(np.diag(a)) < n  # not solvable with diagonal pivots alone
+        lu.factorize(n, n, *_csr(a))
+        assert plug.factorize(_sparse_matrix(a)) and plug.is_factorized()
+        for _ in range(2):
+            b = rng.normal(size=n) + 1j * rng.normal(size=n)
+            x = np.linalg.solve(a, b)
+            x1 = lu.solve(b)
+            x2 = np.array(plug.solve(list(b)))
+            scale = np.abs(x).max()
+            print(f"n={n}: max|x - x_lapack| / max|x| = {np.abs(x1 - x).max() / scale:.2e}")
+            assert np.abs(x1 - x).max() <= 1e-9 * scale
+            assert np.array_equal(x1, x2)
+
+
+def test_lu_singular_matrices(device):
+    """Numerically singular (rank 1) and structurally singular (an empty
+    column) matrices: factorize fails, as the reference's does
+    (cpu_lu_solver.cpp:486-491, 663-668)."""
+    import gap_amd.gap_solver as gs
+    plug = gs.HipLUSolver()
+    for a in (np.array([[1, 2], [2, 4]], complex),
+              np.array([[1, 0, 2], [3, 0, 1], [0, 0, 5]], complex)):
+        with pytest.raises(capi.GapHipError) as e:
+            capi.LU().factorize(a.shape[0], a.shape[0], *_csr(a))
+        assert e.value.code == capi.E_SINGULAR
+        assert not plug.factorize(_sparse_matrix(a)) and not plug.is_factorized()
