@@ -5,6 +5,8 @@
                                      create_gpu_* (include/gap_hip_plugins.h), also
                                      linked as libgap_{admittance,lu,powerflow}_gpu.so
   gap_amd/gap_solver*.so             pybind11 module with the reference's Python surface
+  gap_amd/lib/gap_hip_plugin_check   loads the plugins as the reference factory does and
+                                     solves through the interfaces (integration check)
 
 Run ``python -m gap_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -64,6 +66,9 @@ def build(force: bool = True) -> None:
     # command-line front end (PGM JSON in and out)
     _run(["g++", *CXX_FLAGS, "-I", INC, SRC / "gap_hip_main.cpp", "-o", LIB / "gap_hip",
           "-L", LIB, "-lgap_hip_plugins", "-lgap_hip", "-Wl,-rpath,$ORIGIN"])
+    # plugin-load check: the reference factory's dlopen / dlsym path, linked
+    # against nothing of this project (finds the plugins by bare name)
+    _run(["g++", *CXX_FLAGS, "-I", INC, SRC / "plugin_check.cpp", "-o", LIB / "gap_hip_plugin_check", "-ldl"])
     for alias in ("libgap_admittance_gpu.so", "libgap_lu_gpu.so", "libgap_powerflow_gpu.so"):
         dst = LIB / alias
         if dst.exists() or dst.is_symlink():
