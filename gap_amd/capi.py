@@ -84,6 +84,7 @@ SIGNATURES = {
     "gaphip_plan_set_branch_rx": (C.c_int, [C.c_void_p, C.c_int, _d, _d]),
     "gaphip_plan_set_ybus": (C.c_int, [C.c_void_p, C.c_int, _i, _i, _d, C.c_int, C.c_int]),
     "gaphip_plan_set_schedule": (C.c_int, [C.c_void_p, C.c_int, _i]),
+    "gaphip_plan_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "gaphip_plan_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "gaphip_plan_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "gaphip_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), _d, _i, _i, _d]),
@@ -222,6 +223,11 @@ class Plan:
         _check(lib().gaphip_plan_set_schedule(self._h, int(schedule), C.byref(eff)), "set_schedule")
         self.schedule = {SCHED_LEVELS: "levels", SCHED_GRID: "grid"}[eff.value]
         return self.schedule
+
+    def set_graph(self, on: bool = True):
+        """Level schedule: replay the Newton step from a captured HIP graph (True,
+        the default) or launch its kernels one by one (False)."""
+        _check(lib().gaphip_plan_set_graph(self._h, int(bool(on))), "set_graph")
 
     def close(self):
         if getattr(self, "_h", None):

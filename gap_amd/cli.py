@@ -47,6 +47,10 @@ def main(argv=None) -> int:
     ap.add_argument("-m", "--max-iter", type=int, default=100)
     ap.add_argument("--base-power", type=float, default=None, help="VA (default 1e6 for PGM input, 1e8 synthetic)")
     ap.add_argument("--repeat", type=int, default=1, help="time this many solves after the first")
+    ap.add_argument("--graph", choices=["default", "on", "off"], default="default",
+                    help="level schedule: replay the Newton step from a HIP graph, or launch its kernels one by one")
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="after the solves, copy a 512 MiB device buffer 3 times (calibrates HBM byte counters)")
     args = ap.parse_args(argv)
     if bool(args.input) == bool(args.synthetic):
         ap.error("give an input file or --synthetic")
@@ -66,6 +70,8 @@ def main(argv=None) -> int:
 
     t0 = time.perf_counter()
     plan = capi.Plan(nets, args.precision, schedule=args.schedule)
+    if args.graph != "default":
+        plan.set_graph(args.graph == "on")
     t_plan = time.perf_counter() - t0
     t0 = time.perf_counter()
     res = plan.solve(args.tolerance, args.max_iter, base)
@@ -85,6 +91,13 @@ def main(argv=None) -> int:
         "final_mismatch_max": float(res["final_mismatch"].max()),
     }
     print(json.dumps(summary), flush=True)
+    if args.probe_copy:
+        import torch
+        src = torch.ones(512 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     if args.output:
         out = {"summary": summary, "networks": []}
         for k, net in enumerate(nets):

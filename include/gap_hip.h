@@ -155,6 +155,19 @@ int gaphip_plan_set_ybus(gaphip_plan* plan, int inst, const int* row_ptr, const 
 enum { GAPHIP_SCHED_AUTO = 0, GAPHIP_SCHED_LEVELS = 1, GAPHIP_SCHED_GRID = 2 };
 int gaphip_plan_set_schedule(gaphip_plan* plan, int schedule, int* effective);
 
+/* Launch mode of the level schedule's Newton step (Jacobian, factorisation,
+ * front solves, update: hundreds of launches per iteration on a large grid).
+ * on = 1: the step is captured once per plan into a HIP graph and the graph
+ * is replayed every iteration, one launch per step; on = 0: the kernels are
+ * launched one by one.  Default: on, except for plans with a front of 4,096
+ * slots or more (their look-ahead overlaps two streams, which a replayed
+ * graph overlaps less).  Results are the same bit for bit; the
+ * mismatch and convergence check stay direct launches (the host reads the
+ * count of networks still iterating after them).  Profiling
+ * (gaphip_profile_enable) uses direct launches.  No counterpart in the
+ * reference. */
+int gaphip_plan_set_graph(gaphip_plan* plan, int on);
+
 /* Launch shape of the grid schedule.  A network whose elimination tree holds at
  * least grid_big_work front slots plus factor entries gets grid_big_waves
  * wavefronts per workgroup, the others grid_waves (each 1, 2, 4 or 8); the two
