@@ -24,7 +24,12 @@ Also reported:
     that stratified sample (or, with --no-cpu, on the committed oracle fixture
     of the batch), and the number of grids whose iteration count differs.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grids 4096]
+Before timing, gap_amd.tuning.autotune_plan builds the batch's plan for each
+plan-time candidate (supernode amalgamation thresholds, the grid tasks'
+smallest lane class and packing), times its launch-shape candidates, and keeps
+the fastest plan (`autotune` in the output; --no-tune keeps the defaults).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grids 4096] [--no-tune]
 """
 from __future__ import annotations
 
@@ -65,6 +70,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0,
                     help="seconds of reference CPU solves for cpu_baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="use the default plan options and launch shape")
     ap.add_argument("--probe-copy", action="store_true",
                     help="after the run, copy a 512 MiB device tensor 3 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -74,7 +80,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from gap_amd import capi, gridgen
+    from gap_amd import capi, gridgen, tuning
 
     if not torch.cuda.is_available() or not capi.device_available():
         raise SystemExit("bench.py needs a gfx950 GPU and the built HIP library")
@@ -86,10 +92,18 @@ def main():
     sizes = np.array([n.n_bus for n in nets])
     log(f"[bench] generated {len(nets)} grids, {sizes.sum()} buses in {time.time() - t0:.1f}s")
     t0 = time.time()
-    plan = capi.Plan(nets, "mixed")
+    if args.no_tune:
+        plan = capi.Plan(nets, "mixed")
+        tuned = None
+    else:
+        # plan-time options (amalgamation, grid task layout) and the launch
+        # shape chosen by timing solves of this batch; outside the timed region
+        tuned = tuning.autotune_plan(nets, "mixed", tolerance=tol, max_iterations=max_it, base_power=base,
+                                     repeats=2)
+        plan = tuned["plan"]
     st = plan.stats()
     log(f"[bench] plan: {time.time() - t0:.1f}s (analysis {st['analysis_ms']:.0f} ms), fronts {st['n_fronts']}, "
-        f"levels {st['n_levels']}, max front {st['max_front']}")
+        f"levels {st['n_levels']}, max front {st['max_front']}; options {plan.options}, tuning {plan.tuning()}")
 
     for _ in range(args.warmup):
         r = plan.solve(tol, max_it, base, fetch_v=False)
@@ -215,6 +229,11 @@ def main():
                    "grids": len(nets), "buses": int(sizes.sum()), "tolerance_pu": tol,
                    "base_power_va": base, "max_iterations": max_it},
         "schedule": plan.schedule,
+        "autotune": None if tuned is None else {
+            "options": tuned["options"], "tuning": tuned["tuning"], "ms_per_solve": round(tuned["ms"], 3),
+            "candidates": [{"options": o, "best_tuning": {k: t[k] for k in ("grid_waves", "grid_big_waves",
+                                                                              "grid_big_work")},
+                            "ms_per_solve": round(ms, 3)} for o, t, ms in tuned["table"]]},
         "plain_fp64_solves_per_s": round(plain, 2),
         "plain_fp64_by_schedule": {k: round(v, 2) for k, v in single.items()},
         "speedup_vs_plain_fp64": round(value / plain, 2),

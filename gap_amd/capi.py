@@ -60,7 +60,19 @@ class PlanStats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("grid_waves", C.c_int), ("grid_big_waves", C.c_int), ("grid_big_work", C.c_longlong)]
+    _fields_ = [("grid_waves", C.c_int), ("grid_big_waves", C.c_int), ("grid_big_work", C.c_longlong),
+                ("level_graph", C.c_int), ("big_outer", C.c_int), ("big_two_level_min", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PlanOptions(C.Structure):
+    """Plan-time choices (gaphip_plan_options): supernode amalgamation and the
+    grid schedule's task layout."""
+    _fields_ = [("relax_abs", C.c_int), ("relax_frac", C.c_double), ("grid_min_class", C.c_int),
+                ("grid_pack", C.c_int)]
+    DEFAULTS = {"relax_abs": 4, "relax_frac": 0.10, "grid_min_class": 0, "grid_pack": 1}
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -77,6 +89,8 @@ SIGNATURES = {
     "gaphip_device_available": (C.c_int, []),
     "gaphip_ybus_build": (C.c_int, [C.POINTER(GaphipNet), _i, _i, _d, C.c_int, _i]),
     "gaphip_plan_create": (C.c_int, [C.c_int, C.POINTER(GaphipNet), C.c_int, C.POINTER(C.c_void_p)]),
+    "gaphip_plan_create_opt": (C.c_int, [C.c_int, C.POINTER(GaphipNet), C.c_int, C.POINTER(PlanOptions),
+                                         C.POINTER(C.c_void_p)]),
     "gaphip_plan_destroy": (C.c_int, [C.c_void_p]),
     "gaphip_analyze": (C.c_int, [C.c_int, C.POINTER(GaphipNet), C.POINTER(PlanStats)]),
     "gaphip_plan_stats_get": (C.c_int, [C.c_void_p, C.POINTER(PlanStats)]),
@@ -194,7 +208,8 @@ def ybus(net: Network):
 class Plan:
     """A batch of networks solved together on the GPU (one plan, packed)."""
 
-    def __init__(self, nets: Sequence[Network], precision: int | str = FP64, schedule: int | str = SCHED_AUTO):
+    def __init__(self, nets: Sequence[Network], precision: int | str = FP64, schedule: int | str = SCHED_AUTO,
+                 options: dict | None = None):
         if isinstance(precision, str):
             precision = {"fp64": FP64, "mixed": MIXED}[precision]
         self.nets = list(nets)
@@ -206,7 +221,10 @@ class Plan:
             arr[k] = s
             self._keep.append(keep)
         h = C.c_void_p()
-        _check(lib().gaphip_plan_create(len(self.nets), arr, int(precision), C.byref(h)), "plan_create")
+        self.options = dict(PlanOptions.DEFAULTS, **(options or {}))
+        po = PlanOptions(**self.options)
+        _check(lib().gaphip_plan_create_opt(len(self.nets), arr, int(precision), C.byref(po), C.byref(h)),
+               "plan_create")
         self._h = h
         self._keep = None
         self.bus_off = np.concatenate([[0], np.cumsum([n.n_bus for n in self.nets])]).astype(np.int64)
@@ -249,11 +267,16 @@ class Plan:
         _check(lib().gaphip_plan_get_tuning(self._h, C.byref(t)), "get_tuning")
         return t.as_dict()
 
-    def set_tuning(self, grid_waves=None, grid_big_waves=None, grid_big_work=None) -> dict:
+    def set_tuning(self, grid_waves=None, grid_big_waves=None, grid_big_work=None, **more) -> dict:
+        """Change run-time knobs (gaphip_tuning); unnamed ones keep their value."""
         cur = self.tuning()
-        t = Tuning(cur["grid_waves"] if grid_waves is None else int(grid_waves),
-                   cur["grid_big_waves"] if grid_big_waves is None else int(grid_big_waves),
-                   cur["grid_big_work"] if grid_big_work is None else int(grid_big_work))
+        given = {"grid_waves": grid_waves, "grid_big_waves": grid_big_waves, "grid_big_work": grid_big_work, **more}
+        for k, v in given.items():
+            if k not in cur:
+                raise KeyError(k)
+            if v is not None:
+                cur[k] = int(v)
+        t = Tuning(**cur)
         _check(lib().gaphip_plan_set_tuning(self._h, C.byref(t)), "set_tuning")
         return t.as_dict()
 

@@ -216,6 +216,12 @@ struct DevFronts {
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     int n_fronts = 0, n_levels = 0;
     std::vector<int> level_ptr;  // host copy of the level schedule
+    // large fronts: outer panel width and the front size from which the
+    // two-level panels apply (gaphip_tuning::big_outer, big_two_level_min)
+    int big_outer = kBigOuter, big_two_level_min = kTwoLevelMin;
+    // grid task layout (gaphip_plan_options::grid_min_class, grid_pack)
+    int grid_min_class = 0;
+    bool grid_pack = true;
     // fronts of level l and class c: ids [cls_ptr[l*K+c], cls_ptr[l*K+c+1])
     std::vector<int> cls_ptr;
     std::vector<long long> cls_arrow, cls_factor, cls_slots;  // per (level, class) totals
@@ -343,8 +349,9 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
             for (int n = d.cls_ptr[b]; n < d.cls_ptr[b + 1]; ++n) {
                 int const g = meta[n].inst;
                 if (l >= plans[g]->n_levels) throw Error(GAPHIP_E_ARG, "front level outside its network's tree");
-                byc[lvl_beg[g] + l][c].push_back(n);
-                if (c == kGridClasses - 1) m4 = std::max(m4, meta[n].m);
+                int const ce = std::max(c, d.grid_min_class);
+                byc[lvl_beg[g] + l][ce].push_back(n);
+                if (ce == kGridClasses - 1) m4 = std::max(m4, meta[n].m);
             }
         }
     d.fz_ld4 = m4 + 1;
@@ -367,7 +374,7 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
                 int const beg = static_cast<int>(tfront.size());
                 int cnt = 0;
                 for (; i < F.size() && cnt < per_task; ++i, ++cnt) tfront.push_back(F[i]);
-                for (int c2 = c - 1; c2 >= 0 && cnt < per_task; --c2) {
+                for (int c2 = c - 1; d.grid_pack && c2 >= 0 && cnt < per_task; --c2) {
                     auto& F2 = bc[c2];
                     for (; !F2.empty() && cnt < per_task; ++cnt) {
                         tfront.push_back(F2.back());
@@ -742,8 +749,8 @@ struct FrontRunner {
         // its traffic to dominate (below kTwoLevelMin slots the extra row-block
         // updates and solves cost more than they save): outer width 64 is the
         // plain right-looking scheme.
-        int const outer = mmax >= kTwoLevelMin ? kBigOuter : kPanel;
-        bool const look = outer == kBigOuter && d.side && d.ev_a && d.ev_b;
+        int const outer = mmax >= d.big_two_level_min ? std::max(d.big_outer, kPanel) : kPanel;
+        bool const look = outer > kPanel && d.side && d.ev_a && d.ev_b;
         bool pending = false;  // a side-stream update awaits its join
         for (int j = 0; j < pmax; j += outer) {
             int const jlim = std::min(j + outer, pmax);
@@ -987,7 +994,8 @@ struct gaphip_plan {
     gaphip::DBuf<int> bus_off_d;  // per-instance bus offsets for the grid schedule
     gaphip::DBuf<int> var_off_d;  // per-instance first Newton equation (n_inst + 1)
     int schedule = GAPHIP_SCHED_AUTO;
-    gaphip_tuning tuning{2, 8, -1};
+    gaphip_tuning tuning{2, 8, -1, 1, gaphip::kBigOuter, gaphip::kTwoLevelMin};
+    gaphip_plan_options opts{4, 0.10, 0, 1};
 
     bool grid_schedule() const { return fr.fz_ok && schedule != GAPHIP_SCHED_LEVELS; }
 
@@ -1013,8 +1021,11 @@ struct gaphip_plan {
     hipStream_t stream2 = nullptr;  // second stream of the grid schedule (default priority)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     unsigned long long stagger_ticks = 0;
-    bool use_graph = true;                  // level schedule: replay the captured Newton step
-    hipGraphExec_t newton_graph = nullptr;  // captured on the first solve that needs it  // hold of the grid schedule's second launch (kStaggerUs in wall-clock ticks)
+    hipGraphExec_t newton_graph = nullptr;  // captured on the first solve that needs it (tuning.level_graph)
+    void drop_graph() {
+        if (newton_graph) (void)hipGraphExecDestroy(newton_graph);
+        newton_graph = nullptr;
+    }  // hold of the grid schedule's second launch (kStaggerUs in wall-clock ticks)
 
     ~gaphip_plan() {
         if (newton_graph) (void)hipGraphExecDestroy(newton_graph);
@@ -1054,7 +1065,10 @@ void analyze(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precision) 
                 return;
             }
             LUInput in = nr_lu_input(P.nets[g], P.yp[g], P.np[g], P.node_bus[g]);
-            P.lp[g] = plan_lu(in);
+            LUOptions lo;
+            lo.relax_abs = P.opts.relax_abs;
+            lo.relax_frac = P.opts.relax_frac;
+            P.lp[g] = plan_lu(in, lo);
         } catch (std::exception const& e) {
             errs[g] = e.what();
             bad[g] = 1;
@@ -1323,7 +1337,7 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     // solve) and 0.6% slower on the 100k-bus grid, whose two-level fronts
     // overlap a side stream that the replayed graph serialises more: direct
     // launches there
-    P.use_graph = P.fr.max_front < kTwoLevelMin;
+    P.tuning.level_graph = P.fr.max_front < P.tuning.big_two_level_min ? 1 : 0;
     P.analysis_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1455,6 +1469,8 @@ void run_solve(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front, T* v
     hipStream_t s = P.stream;
     int const G = P.n_inst;
     int const max_it = o.max_iterations;
+    P.fr.big_outer = P.tuning.big_outer;
+    P.fr.big_two_level_min = P.tuning.big_two_level_min;
     if (P.sspec_dirty || P.sspec_base != o.base_power) upload_sspec(P, o.base_power);
     int rows = max_it + 1;
     if (P.maxf_rows < rows) {
@@ -1492,7 +1508,7 @@ void run_solve(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front, T* v
         HIPCHK(hipEventRecord(P.ev, s));
         HIPCHK(hipEventSynchronize(P.ev));
         if (*P.pinned_active == 0) break;
-        if (P.use_graph && !P.prof.on) {
+        if (P.tuning.level_graph && !P.prof.on) {
             if (!P.newton_graph) capture_newton_step<T>(P, jv, front, vec);
             HIPCHK(hipGraphLaunch(P.newton_graph, s));
         } else {
@@ -1861,18 +1877,35 @@ int gaphip_device_available(void) {
     return std::strstr(prop.gcnArchName, "gfx950") != nullptr ? 1 : 0;
 }
 
-int gaphip_plan_create(int n_inst, const gaphip_net* nets, int precision, gaphip_plan** out) {
+int gaphip_plan_create_opt(int n_inst, const gaphip_net* nets, int precision, const gaphip_plan_options* options,
+                           gaphip_plan** out) {
     GAPHIP_GUARD({
         if (!out) throw Error(GAPHIP_E_ARG, "out is null");
         *out = nullptr;
         auto p = std::make_unique<gaphip_plan>();
+        if (options) {
+            if (options->relax_abs < 0 || !(options->relax_frac >= 0.0) || options->grid_min_class < 0 ||
+                options->grid_min_class >= kGridClasses)
+                throw Error(GAPHIP_E_ARG, "plan options out of range");
+            p->opts = *options;
+        }
+        p->fr.grid_min_class = p->opts.grid_min_class;
+        p->fr.grid_pack = p->opts.grid_pack != 0;
         build_plan(*p, n_inst, nets, precision);
         // default launch shape (measured on the 4,096-grid batch): with FP32
         // factors the biggest networks get 4 wavefronts; the FP64 kernel holds
         // more registers per wavefront and runs best with 2 throughout
-        if (precision == GAPHIP_MIXED) p->tuning = gaphip_tuning{2, 4, 40000};
+        if (precision == GAPHIP_MIXED) {
+            p->tuning.grid_waves = 2;
+            p->tuning.grid_big_waves = 4;
+            p->tuning.grid_big_work = 40000;
+        }
         *out = p.release();
     })
+}
+
+int gaphip_plan_create(int n_inst, const gaphip_net* nets, int precision, gaphip_plan** out) {
+    return gaphip_plan_create_opt(n_inst, nets, precision, nullptr, out);
 }
 
 int gaphip_analyze(int n_inst, const gaphip_net* nets, gaphip_plan_stats* o) {
@@ -1992,7 +2025,8 @@ int gaphip_plan_set_schedule(gaphip_plan* P, int schedule, int* effective) {
 int gaphip_plan_set_graph(gaphip_plan* P, int on) {
     GAPHIP_GUARD({
         if (!P) throw Error(GAPHIP_E_ARG, "null plan");
-        P->use_graph = on != 0;
+        P->tuning.level_graph = on != 0;
+        P->drop_graph();
     })
 }
 
@@ -2009,7 +2043,12 @@ int gaphip_plan_set_tuning(gaphip_plan* P, const gaphip_tuning* t) {
         auto ok = [](int w) { return w == 1 || w == 2 || w == 4 || w == 8; };
         if (!ok(t->grid_waves) || !ok(t->grid_big_waves))
             throw Error(GAPHIP_E_ARG, "wavefronts per workgroup must be 1, 2, 4 or 8");
+        if (t->big_outer < kBigPanel || t->big_outer > 1024 || t->big_outer % kBigPanel != 0)
+            throw Error(GAPHIP_E_ARG, "big_outer must be a multiple of 64 between 64 and 1024");
+        if (t->big_two_level_min < 0 || (t->level_graph != 0 && t->level_graph != 1))
+            throw Error(GAPHIP_E_ARG, "big_two_level_min must be >= 0 and level_graph 0 or 1");
         P->tuning = *t;
+        P->drop_graph();  // the captured step holds the panel widths
     })
 }
 

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define GAPHIP_ABI_VERSION 3
+#define GAPHIP_ABI_VERSION 4
 
 enum {
     GAPHIP_OK = 0,
@@ -105,6 +105,26 @@ typedef struct gaphip_plan_stats {
 } gaphip_plan_stats;
 
 int gaphip_plan_create(int n_inst, const gaphip_net* nets, int precision, gaphip_plan** out);
+
+/* Plan-time choices of the symbolic analysis and the grid schedule's task
+ * layout (a plan must be created again to change them); results agree to
+ * rounding whatever the choice.  relax_abs / relax_frac: relaxed supernode
+ * amalgamation -- a child supernode that precedes its parent in postorder is
+ * merged while the explicit zeros added stay within
+ * max(relax_abs, relax_frac * merged front entries), in node units (defaults
+ * 4, 0.10).  grid_min_class: smallest lanes-per-front class of a grid task, 0
+ * to 4 for 4, 8, 16, 32, 64 lanes (default 0).  grid_pack: 1 (default) lets
+ * smaller fronts fill the free lane groups of a larger class's last task of a
+ * level.  No counterpart in the reference. */
+typedef struct gaphip_plan_options {
+    int relax_abs;
+    double relax_frac;
+    int grid_min_class;
+    int grid_pack;
+} gaphip_plan_options;
+/* gaphip_plan_create with the given options (NULL: the defaults). */
+int gaphip_plan_create_opt(int n_inst, const gaphip_net* nets, int precision, const gaphip_plan_options* options,
+                           gaphip_plan** out);
 /* Host-only symbolic analysis of a batch (no device touched): fills the same
  * statistics as gaphip_plan_stats_get so fill, front memory and the level
  * count can be sized before anything is allocated. */
@@ -168,19 +188,29 @@ int gaphip_plan_set_schedule(gaphip_plan* plan, int schedule, int* effective);
  * reference. */
 int gaphip_plan_set_graph(gaphip_plan* plan, int on);
 
-/* Launch shape of the grid schedule.  A network whose elimination tree holds at
- * least grid_big_work front slots plus factor entries gets grid_big_waves
+/* Run-time schedule knobs of a plan; results do not depend on them.
+ * Grid schedule: a network whose elimination tree holds at least
+ * grid_big_work front slots plus factor entries gets grid_big_waves
  * wavefronts per workgroup, the others grid_waves (each 1, 2, 4 or 8); the two
  * groups run concurrently, the second launched a few microseconds after the
- * first.  grid_big_work < 0 puts every network in the first group.  Results
- * do not depend on the tuning.  Defaults: 2, 4, 40000 for GAPHIP_MIXED plans,
- * 2, 8, -1 for GAPHIP_FP64 plans.
+ * first.  grid_big_work < 0 puts every network in the first group.
+ * Level schedule: level_graph 1 replays the Newton step from a captured HIP
+ * graph, 0 launches its kernels one by one (see gaphip_plan_set_graph);
+ * fronts of at least big_two_level_min slots are factored in outer panels of
+ * big_outer pivots (a multiple of 64, at most 1024), each as 64-wide inner
+ * panels with one trailing update of K = big_outer per outer panel, and the
+ * next panel's update overlapped; smaller fronts above 128 slots use 64-wide
+ * panels.  Defaults: 2, 4, 40000 (GAPHIP_MIXED) or 2, 8, -1 (GAPHIP_FP64); graph
+ * on unless a front reaches big_two_level_min; 256; 4096.
  * gap_amd.tuning.autotune() times candidates on a plan and sets the fastest.
  * No counterpart in the reference. */
 typedef struct gaphip_tuning {
     int grid_waves;
     int grid_big_waves;
     long long grid_big_work;
+    int level_graph;
+    int big_outer;
+    int big_two_level_min;
 } gaphip_tuning;
 int gaphip_plan_get_tuning(const gaphip_plan* plan, gaphip_tuning* out);
 int gaphip_plan_set_tuning(gaphip_plan* plan, const gaphip_tuning* tuning);

@@ -37,7 +37,7 @@ def test_plugin_factories_exported(so):
 def test_abi_version_and_no_device_paths():
     from gap_amd import capi
     L = capi.lib()
-    assert L.gaphip_abi_version() == 3
+    assert L.gaphip_abi_version() == 4
     if capi.device_available():
         pytest.skip("a device is visible; the no-device error path is exercised elsewhere")
     net = __import__("gap_amd.gridgen", fromlist=["x"]).small_mesh(5, seed=0)

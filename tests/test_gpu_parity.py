@@ -236,7 +236,8 @@ def test_grid_tuning_does_not_change_results(device):
     ref = p.solve(1e-8, 100, 100e6)
     for gw, bw, work in [(1, 8, 20_000), (4, 4, -1), (8, 2, 0), (2, 8, 40_000)]:
         p.set_tuning(gw, bw, work)
-        assert p.tuning() == {"grid_waves": gw, "grid_big_waves": bw, "grid_big_work": work}
+        t = p.tuning()
+        assert (t["grid_waves"], t["grid_big_waves"], t["grid_big_work"]) == (gw, bw, work)
         r = p.solve(1e-8, 100, 100e6)
         assert np.array_equal(r["iterations"], ref["iterations"])
         assert all(np.array_equal(a, b) for a, b in zip(r["v"], ref["v"]))
@@ -247,6 +248,58 @@ def test_grid_tuning_does_not_change_results(device):
     assert p.tuning() == out["tuning"] and len(out["table"]) == 4
     r = p.solve(1e-8, 100, 100e6)
     assert all(np.array_equal(a, b) for a, b in zip(r["v"], ref["v"]))
+
+
+def test_level_tuning_does_not_change_results(device):
+    """The level schedule's knobs -- graph replay, the large fronts' outer panel
+    width and two-level threshold -- change no result bit (a 3,000-bus grid
+    with a root front above 128 slots; the threshold is lowered so that its
+    fronts take the two-level path)."""
+    from gap_amd import tuning
+    big = gridgen.large_mesh(3000, seed=2, n_extra=900, n_trafo=300, n_shunt=60)
+    p = capi.Plan([big], precision="mixed", schedule="levels")
+    ref = p.solve(1e-8, 100, 100e6)
+    for knobs in ({"level_graph": 0, "big_outer": 128, "big_two_level_min": 256},
+                  {"level_graph": 1, "big_outer": 512, "big_two_level_min": 256},
+                  {"level_graph": 1, "big_outer": 64, "big_two_level_min": 0},
+                  {"level_graph": 0, "big_outer": 256, "big_two_level_min": 4096}):
+        p.set_tuning(**knobs)
+        r = p.solve(1e-8, 100, 100e6)
+        assert np.array_equal(r["iterations"], ref["iterations"]), knobs
+        assert np.abs(r["v"][0] - ref["v"][0]).max() <= 1e-10, knobs
+    for bad in ({"big_outer": 100}, {"big_outer": 2048}, {"big_two_level_min": -1}, {"level_graph": 2}):
+        with pytest.raises(capi.GapHipError):
+            p.set_tuning(**bad)
+    out = tuning.autotune(p, candidates={"level_graph": (0, 1), "big_outer": (256,), "big_two_level_min": (4096,)},
+                          repeats=1)
+    assert len(out["table"]) == 2 and p.tuning() == out["tuning"]
+
+
+def test_plan_options_and_autotune_plan(device, oracle):
+    """Plan-time choices (amalgamation thresholds, the grid tasks' smallest lane
+    class and packing) give the same iteration counts and voltages within
+    1e-10 of each other and of the oracle within 1e-8; the plan autotuner keeps
+    the fastest plan with its options."""
+    from gap_amd import tuning
+    nets = _batch(8, 77)
+    ref = capi.Plan(nets, precision="mixed").solve(1e-8, 100, 100e6)
+    for opts in ({"grid_pack": 0}, {"grid_min_class": 1}, {"grid_min_class": 2, "grid_pack": 0},
+                 {"relax_abs": 2, "relax_frac": 0.05}, {"relax_abs": 8, "relax_frac": 0.2}):
+        p = capi.Plan(nets, precision="mixed", options=opts)
+        assert p.options == dict(capi.PlanOptions.DEFAULTS, **opts)
+        r = p.solve(1e-8, 100, 100e6)
+        assert np.array_equal(r["iterations"], ref["iterations"]), opts
+        assert max(np.abs(a - b).max() for a, b in zip(r["v"], ref["v"])) <= 1e-10, opts
+    with pytest.raises(capi.GapHipError):
+        capi.Plan(nets, options={"grid_min_class": 5})
+    out = tuning.autotune_plan(nets, "mixed", plan_candidates=({}, {"grid_pack": 0}),
+                               candidates={"grid_waves": (2,), "grid_big_waves": (4,), "grid_big_work": (-1,)},
+                               repeats=1)
+    assert len(out["table"]) == 2 and out["plan"].options == out["options"]
+    r = out["plan"].solve(1e-8, 100, 100e6)
+    for k, net in enumerate(nets[:3]):
+        o = oracle.solve(net, tol=1e-8, max_it=100, j_call=-1)
+        assert int(r["iterations"][k]) == o["iterations"] and np.abs(r["v"][k] - o["v"]).max() <= V_TOL
 
 
 def test_grid_schedule_vs_oracle(device, oracle):
