@@ -2,7 +2,8 @@
 parity tests at full size:
 
   batch4096_oracle_sample.npz  a stratified sample of the 4,096-grid batch;
-  grid10k_oracle.npz           the 10,000-bus meshed grid (about a minute).
+  grid10k_oracle.npz           the 10,000-bus meshed grid (about a minute);
+  batch4096_full_load_oracle.npz  the same batch at load level 1.0.
 
 The batch:
 The batch is gap_amd.gridgen.batch_grids(4096) (seed 1000; sizes log-uniform in
@@ -25,7 +26,17 @@ flag, per-iteration max|F|, voltages and the grid's checksum.  (The reference's
 own CPU path needs dense n_var x n_var arrays, ~4.4 GB and hours at this size,
 so the oracle, pinned to it on the golden cases, stands in.)
 
-Usage:  python tests/golden/make_oracle_fixtures.py [batch] [grid10k]
+The batch at load level 1.0 (gridgen.batch_grids(4096, load_level=1.0), the
+per-grid loads of the 200-bus grid x min(1, 200/n)^1.5 x U[0.7, 1.3] with no
+further scaling): every grid solved by the oracle (8 processes, about 10 s),
+storing per grid the checksum, iteration count, converged flag and final
+max|F|; 12 grids diverge (iterations = the cap of 100).  Voltages are stored for
+a sample: every grid that needs 11 or more iterations (the most heavily loaded
+that still converge), the 10 largest and 30 evenly spaced in size.  The two
+smallest diverging grids (123 and 136 buses) are also solved by the
+reference's own CPU sources, which must diverge the same way.
+
+Usage:  python tests/golden/make_oracle_fixtures.py [batch] [grid10k] [full_load]
 """
 from __future__ import annotations
 
@@ -45,6 +56,7 @@ import pyoracle  # noqa: E402
 HERE = Path(__file__).resolve().parent
 OUT = HERE / "batch4096_oracle_sample.npz"
 OUT10K = HERE / "grid10k_oracle.npz"
+OUTFULL = HERE / "batch4096_full_load_oracle.npz"
 TOL, MAX_IT, BASE = 1e-8, 100, 100e6
 
 
@@ -104,9 +116,54 @@ def make_batch():
           f"({OUT.stat().st_size / 1e6:.1f} MB)")
 
 
+def _solve_full_load(k):
+    """Oracle solve of grid k of batch_grids(4096, load_level=1.0), built alone
+    (the same seeds and sizes as the whole batch)."""
+    sizes = gridgen.batch_sizes(4096)
+    f = np.random.default_rng(1000 + 1 + k).uniform(0.7, 1.3)
+    net = gridgen.small_mesh(int(sizes[k]), seed=1000 + 1 + k, n_extra=40, load_factor=float(f) * 1.0)
+    r = pyoracle.Oracle().solve(net, tol=TOL, max_it=MAX_IT, base_power=BASE, j_call=-1, traj=False)
+    return k, grid_checksum(net), r["iterations"], r["converged"], r["final_mismatch"], r["v"]
+
+
+def make_full_load():
+    from multiprocessing import Pool
+    t0 = time.time()
+    with Pool(8) as pool:
+        res = sorted(pool.map(_solve_full_load, range(4096), chunksize=16), key=lambda r: r[0])
+    sizes = gridgen.batch_sizes(4096)
+    its = np.array([r[2] for r in res], np.int32)
+    conv = np.array([r[3] for r in res])
+    order = np.argsort(sizes, kind="stable")
+    sample = np.unique(np.concatenate([np.nonzero(conv & (its >= 11))[0], order[-10:],
+                                       order[np.linspace(0, 4095, 30).round().astype(int)]]))
+    vs = [res[i][5] for i in sample]
+    off = np.concatenate([[0], np.cumsum([len(v) for v in vs])])
+    div = np.nonzero(~conv)[0]
+    n_ref = 0
+    if pyoracle.reference_available():
+        ref = pyoracle.Reference()
+        for i in div[np.argsort(sizes[div])][:2]:
+            f = np.random.default_rng(1000 + 1 + int(i)).uniform(0.7, 1.3)
+            net = gridgen.small_mesh(int(sizes[i]), seed=1000 + 1 + int(i), n_extra=40, load_factor=float(f))
+            rr = ref.solve(net, tol=TOL, max_it=MAX_IT, base_power=BASE, j_call=-1, traj=False)
+            assert not rr["converged"] and rr["iterations"] == its[i] == MAX_IT, int(i)
+            n_ref += 1
+    np.savez_compressed(OUTFULL, size=sizes.astype(np.int32), checksum=np.array([r[1] for r in res]),
+                        iterations=its, converged=conv, final_mismatch=np.array([r[4] for r in res]),
+                        sample=sample.astype(np.int32), v=np.concatenate(vs), v_off=off.astype(np.int64),
+                        tolerance=TOL, max_iterations=MAX_IT, base_power=BASE, load_level=1.0)
+    print(f"full load: 4096 grids in {time.time() - t0:.1f}s; {len(div)} diverge (sizes {sorted(sizes[div].tolist())}), "
+          f"{n_ref} of them also with the reference sources; iterations of the converged "
+          f"{dict(zip(*np.unique(its[conv], return_counts=True)))}; V of {len(sample)} grids; wrote {OUTFULL.name} "
+          f"({OUTFULL.stat().st_size / 1e6:.2f} MB)")
+
+
 def main():
     pyoracle.build()
-    which = sys.argv[1:] or ["batch", "grid10k"]
+    which = sys.argv[1:] or ["batch", "grid10k", "full_load"]
+    if "full_load" in which:
+        make_full_load()
     if "batch" in which:
         make_batch()
     if "grid10k" in which:
