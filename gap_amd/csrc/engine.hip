@@ -445,6 +445,35 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
         }
         lvl_ptr.push_back(static_cast<int>(tasks.size()));
     }
+    // Dependencies between the tasks of a network: the task holding the last
+    // child (factorisation) and the first parent (backward substitution) of
+    // any of a task's fronts.  Tasks are in level order and a child's level is
+    // below its parent's, so dep_f < task < dep_b, which the kernel's waits
+    // rely on to end.
+    {
+        std::vector<int> task_of(nf, -1), parent(nf, -1);
+        for (size_t k = 0; k < tasks.size(); ++k)
+            for (int q = tasks[k].front_beg; q < tasks[k].front_beg + (tasks[k].cls_cnt & 0xff); ++q)
+                task_of[tfront[q]] = static_cast<int>(k);
+        for (int n = 0; n < nf; ++n)
+            for (int ci = meta[n].child_beg; ci < meta[n].child_end; ++ci) {
+                if (cinfo[ci].id < 0 || cinfo[ci].id >= nf) throw Error(GAPHIP_E_ARG, "child front out of range");
+                parent[cinfo[ci].id] = n;
+            }
+        for (size_t k = 0; k < tasks.size(); ++k) {
+            GridTask& tk = tasks[k];
+            tk.dep_f = -1;
+            tk.dep_b = 0x7fffffff;
+            for (int q = tk.front_beg; q < tk.front_beg + (tk.cls_cnt & 0xff); ++q) {
+                int const n = tfront[q];
+                for (int ci = meta[n].child_beg; ci < meta[n].child_end; ++ci)
+                    tk.dep_f = std::max(tk.dep_f, task_of[cinfo[ci].id]);
+                if (parent[n] >= 0) tk.dep_b = std::min(tk.dep_b, task_of[parent[n]]);
+            }
+            if (tk.dep_f >= static_cast<int>(k) || tk.dep_b <= static_cast<int>(k))
+                throw Error(GAPHIP_E_ARG, "grid task dependencies out of level order");
+        }
+    }
     // algorithmic bytes per Newton iteration of each instance (factor + backward):
     // fz_fixed[g] bytes of indices and FP64 inputs, fz_welems[g] entries of the
     // factor precision

@@ -21,8 +21,12 @@
 //           list order = child order, so every solve sums in the same order);
 //   vx    : child right-hand-side entries, added likewise;
 //   sol   : (backward) solution values of the structure slots.
-// Inside a task only wavefront-level synchronisation is needed; levels are
-// separated by workgroup barriers.
+// Inside a task only wavefront-level synchronisation is needed.  Between tasks
+// of different wavefronts the order is kept by dependencies, not by a barrier
+// per level: a wavefront publishes in LDS the index of the next task it will
+// run, and a task waits only until the wavefronts that hold its children (in
+// the factorisation) or its parents (in the backward substitution) have moved
+// past them.
 #pragma once
 
 namespace gaphip {
@@ -34,6 +38,8 @@ struct alignas(16) GridTask {
     int ext_beg, ext_end;
     int vx_beg, vx_end;
     int sol_beg, sol_end;
+    int dep_f;  // largest task index holding a child of one of its fronts (-1: none)
+    int dep_b;  // smallest task index holding a parent of one of its fronts (INT_MAX: none)
 };
 
 struct FusedArgs {
@@ -641,29 +647,67 @@ __global__ void __launch_bounds__(256) k_gather_ypair(int n, const int4* pairs, 
     if (k < n) ypair[k] = y[pairs[k].z];
 }
 
-// This wavefront's tasks q, q + NW, ... < qe of one level (q is uniform over
-// the wavefront).  While a task runs, the next task's descriptor-dependent
-// loads and the descriptor after that are in flight (unrolled by two, so no
-// value is copied while its load is outstanding).  The descriptors read past
-// qe are the next level's or padding; only their static lists are loaded.
+// Progress of the wavefronts of a workgroup through a sweep: prog[w] is the
+// index of the next task wavefront w will run (its tasks are w, w + NW, ...
+// of the network's task list, in order).  Tasks are in level order, children
+// before parents, so in the factorisation every task's children have smaller
+// indices (dep_f < index) and in the backward substitution every task's
+// parents larger ones (dep_b > index); the wavefront whose next task comes
+// first (last) in the sweep always finds its dependencies done, so every wait
+// ends.
+__device__ inline void publish_progress(int* prog, int wave, int next) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this task's stores before the new index
+    if (lane_id() == 0) __hip_atomic_store(prog + wave, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <bool BWD, int NW>
+__device__ inline void wait_for_deps(int const* prog, int wave, GridTask const& tk) {
+    int const dep = BWD ? tk.dep_b : tk.dep_f;
+    if (BWD ? dep == 0x7fffffff : dep < 0) return;
+    for (int w = 0; w < NW; ++w) {
+        if (w == wave) continue;
+        for (;;) {
+            int const at = __hip_atomic_load(prog + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (BWD ? at < dep : at > dep) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // then the producers' stores are visible
+}
+
+// This wavefront's tasks of one sweep over the network's task list [lo, hi):
+// ascending from q = lo + wave (factorisation) or descending from
+// q = hi - 1 - wave (backward), NW apart, without a barrier between levels.
+// While a task runs, the next task's descriptor-dependent loads and the
+// descriptor after that are in flight (unrolled by two, so no value is copied
+// while its load is outstanding), across level boundaries.  Descriptors read
+// outside [lo, hi) are another network's or padding; only their static lists
+// are loaded.
 template <typename T, bool BWD, int NW>
-__device__ inline void run_level(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma,
-                                 const GridTask* __restrict__ tasks, int g, int q, int qe,
-                                 T* front, T* vec, T* pool) {
-    if (q >= qe) return;
-    GridTask t0 = tasks[q], t1 = tasks[q + NW];
+__device__ inline void run_sweep(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma,
+                                 const GridTask* __restrict__ tasks, int g, int wave, int lo, int hi, T* front,
+                                 T* vec, T* pool, int* prog) {
+    constexpr int S = BWD ? -NW : NW;
+    int q = BWD ? hi - 1 - wave : lo + wave;
+    auto inside = [&](int k) { return BWD ? k >= lo : k < hi; };
+    auto desc = [&](int k) { return tasks[BWD ? max(k, 0) : k]; };
+    if (!inside(q)) return;
+    GridTask t0 = tasks[q], t1 = desc(q + S);
     TaskPre<T> p0 = load_pre<T, BWD>(a, t0);
     for (;;) {
+        wait_for_deps<BWD, NW>(prog, wave, t0);
         TaskPre<T> const p1 = load_pre<T, BWD>(a, t1);
-        GridTask const t2 = tasks[q + 2 * NW];
+        GridTask const t2 = desc(q + 2 * S);
         grid_task<T, BWD>(a, fa, ma, g, t0, p0, front, vec, pool);
-        q += NW;
-        if (q >= qe) break;
+        q += S;
+        publish_progress(prog, wave, q);
+        if (!inside(q)) break;
+        wait_for_deps<BWD, NW>(prog, wave, t1);
         TaskPre<T> const p2 = load_pre<T, BWD>(a, t2);
-        GridTask const t3 = tasks[q + 2 * NW];
+        GridTask const t3 = desc(q + 2 * S);
         grid_task<T, BWD>(a, fa, ma, g, t1, p1, front, vec, pool);
-        q += NW;
-        if (q >= qe) break;
+        q += S;
+        publish_progress(prog, wave, q);
+        if (!inside(q)) break;
         t0 = t2;
         t1 = t3;
         p0 = p2;
@@ -699,15 +743,20 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1)
     constexpr int NT = NW * kWave;
     extern __shared__ __align__(16) unsigned char grid_smem[];
     __shared__ double red[NW];
+    __shared__ int prog_f[NW], prog_b[NW];  // per wavefront: next task of the factor / backward sweep
     int const g = a.order[a.order_off + blockIdx.x];
     int const b0 = a.bus_off[g], b1 = a.bus_off[g + 1];
-    int const e0 = a.lvl_beg[g], e1 = a.lvl_beg[g + 1];
+    int const t_lo = lvl_ptr[a.lvl_beg[g]], t_hi = lvl_ptr[a.lvl_beg[g + 1]];  // the network's tasks
     int const tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     T* pool = reinterpret_cast<T*>(grid_smem) + wave * a.pool;
     for (int it = 0; it < a.max_it; ++it) {
         // cleared before block_max's barriers, which order it before any
         // factor task of this iteration can set it
         if (tid == 0) a.lu_fail[g] = 0;
+        if ((tid & (kWave - 1)) == 0) {
+            prog_f[wave] = t_lo + wave;
+            prog_b[wave] = t_hi - 1 - wave;
+        }
         double m = 0.0;
         constexpr int MB = sizeof(T) == 8 ? 2 : 1;  // buses per thread and step
         for (int i = b0 + tid; i < b1; i += MB * NT) {
@@ -729,14 +778,11 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1)
             }
             break;
         }
-        for (int e = e0; e < e1; ++e) {
-            run_level<T, false, NW>(a, fa, ma, tasks, g, lvl_ptr[e] + wave, lvl_ptr[e + 1], front, vec, pool);
-            __syncthreads();
-        }
-        for (int e = e1 - 1; e >= e0; --e) {
-            run_level<T, true, NW>(a, fa, ma, tasks, g, lvl_ptr[e] + wave, lvl_ptr[e + 1], front, vec, pool);
-            __syncthreads();
-        }
+        // (block_max's barriers also published the progress counters)
+        run_sweep<T, false, NW>(a, fa, ma, tasks, g, wave, t_lo, t_hi, front, vec, pool, prog_f);
+        __syncthreads();
+        run_sweep<T, true, NW>(a, fa, ma, tasks, g, wave, t_lo, t_hi, front, vec, pool, prog_b);
+        __syncthreads();
         bool const zero = a.lu_fail[g] != 0;
         for (int i = b0 + tid; i < b1; i += NT) grid_update_bus(ua, i, zero);
         __syncthreads();
