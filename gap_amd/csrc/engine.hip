@@ -482,10 +482,12 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     for (size_t k = 0; k < tasks.size(); ++k) {
         GridTask const& tk = tasks[k];
         int const g = meta[tfront[tk.front_beg]].inst;
-        d.fz_fixed[g] += sizeof(GridTask) * 2.0 + 64.0 * (tk.pair_end - tk.pair_beg) + 4.0 * (tk.rhs_end - tk.rhs_beg) +
+        d.fz_fixed[g] += sizeof(GridTask) * 2.0 + 16.0 * (tk.pair_end - tk.pair_beg) + 4.0 * (tk.rhs_end - tk.rhs_beg) +
                          8.0 * (tk.ext_end - tk.ext_beg) + 8.0 * (tk.vx_end - tk.vx_beg) + 16.0 * (tk.sol_end - tk.sol_beg);
         // -F of each right-hand-side entry: written by the mismatch pass, read by the task
         d.fz_welems[g] += 2.0 * (tk.rhs_end - tk.rhs_beg) + (tk.ext_end - tk.ext_beg) + (tk.vx_end - tk.vx_beg);
+        // per pair, in the factor type: its Y, and V_i with V_j or S_i (the Jacobian block's inputs)
+        d.fz_welems[g] += 6.0 * (tk.pair_end - tk.pair_beg);
         for (int q = tk.front_beg; q < tk.front_beg + (tk.cls_cnt & 0xff); ++q) {
             int const n = tfront[q];
             double const m = meta[n].m, pp = meta[n].p;
@@ -1015,6 +1017,7 @@ struct gaphip_plan {
     gaphip::DBuf<int2> brhs;  // grid schedule: right-hand-side list positions of each bus's equations
     gaphip::DBuf<double> adj_lossb, adj_gp, adj_gq, adj_gr, adj_gx, adj_gvm, adj_gva;
     gaphip::DBuf<double2> v, vpol, s, sspec, y_pu, y_given;
+    gaphip::DBuf<float2> vf, sf;  // grid schedule with FP32 factors: V and S in FP32 (allocated on first use)
     std::vector<int> y_given_mode;  // per instance: 0 assemble, 1 siemens given, 2 per-unit given
     gaphip::DBuf<unsigned long long> maxf;
     gaphip::DBuf<double> front64, vec64;
@@ -1633,13 +1636,18 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
 
     if (max_it > 0) {
         DevFronts const& d = P.fr;
-        hipLaunchKernelGGL(k_gather_ypair, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
-                           static_cast<int>(d.fz_npairs), d.fz_pairs.p, P.y_pu.p, d.fz_ypair.p);
+        hipLaunchKernelGGL(k_gather_ypair<T>, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
+                           static_cast<int>(d.fz_npairs), d.fz_pairs.p, P.y_pu.p,
+                           reinterpret_cast<cplx_t<T>*>(d.fz_ypair.p));
+        if (sizeof(T) == 4 && !P.vf.p) {
+            P.vf.alloc(N);
+            P.sf.alloc(N);
+        }
         FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, P.var_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
                      d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_frhs.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
                      P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, P.dx.p};
         MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
-                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p, P.brhs.p, d.fz_frhs.p};
+                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p, P.brhs.p, d.fz_frhs.p, P.vf.p, P.sf.p};
         UpdArgs ua{static_cast<int>(N), P.bus_inst.p, P.done.p, P.lu_fail.p, P.ang.p, P.mag.p, P.dx.p, P.v.p};
         FrontArgs fra = d.args(nullptr, P.lu_fail.p);
         gaphip_tuning const& tu = P.tuning;
@@ -1679,9 +1687,11 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
             double const n = P.bus_off[g + 1] - P.bus_off[g], z = P.y_off[g + 1] - P.y_off[g];
             double const v = P.var_off[g + 1] - P.var_off[g];
             double const mis = 20.0 * z + 56.0 * n + 8.0 * v;
-            double const full = mis + P.fr.fz_fixed[g] + w * P.fr.fz_welems[g] + (8.0 * v + 32.0 * n);
+            // the grid mismatch pass with FP32 factors also writes V and S in FP32
+            double const mis_f32 = sizeof(T) == 4 ? 16.0 * n : 0.0;
+            double const full = mis + mis_f32 + P.fr.fz_fixed[g] + w * P.fr.fz_welems[g] + (8.0 * v + 32.0 * n);
             int const newton = done[g] ? its[g] - 1 : max_it;
-            total += newton * full + (done[g] ? mis : 0.0);
+            total += newton * full + (done[g] ? mis + mis_f32 : 0.0);
         }
         P.prof.bytes[K_FUSED] += total;
     }

@@ -55,7 +55,7 @@ struct FusedArgs {
     const GridTask* tasks;
     const int4* gfront;   // per front: (storage offset, vector offset, m | p << 16, first slot)
     const int4* pairs;    // (i, j, Y slot, LDS position | entry mask << 16)
-    const double2* ypair; // per-unit Y value of each pair, in pair order (gathered once per solve)
+    const void* ypair;    // per-unit Y value of each pair (the factor type's complex), in pair order (gathered once per solve)
     const void* frhs;     // per right-hand-side entry: -F (factor type), written by the mismatch pass
     const int2* rhs;      // (equation index, LDS position)
     const int2* ext;      // (front storage index, LDS position)
@@ -103,6 +103,19 @@ __device__ inline double2 ld_stream(const double2* p) {
     const double* q = reinterpret_cast<const double*>(p);
     return make_double2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
 }
+__device__ inline float2 ld_stream(const float2* p) {
+    const float* q = reinterpret_cast<const float*>(p);
+    return make_float2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+}
+
+// Complex of the factor type: the Jacobian blocks of the grid kernel are
+// evaluated in it (FP32 factors: from FP32 copies of V, S and Y, since every
+// entry is rounded to FP32 for the factorisation anyway; the mismatch, the
+// update and the convergence test stay FP64).
+template <typename T> struct Cplx;
+template <> struct Cplx<float> { using type = float2; };
+template <> struct Cplx<double> { using type = double2; };
+template <typename T> using cplx_t = typename Cplx<T>::type;
 
 // Lane index, recomputed at each call: an empty volatile asm keeps the
 // compiler from hoisting the lane-dependent masks and addresses of every task
@@ -203,6 +216,10 @@ __device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B]
         double const sr = vi.x * cr[b] + vi.y * cim[b];
         double const si = vi.y * cr[b] - vi.x * cim[b];
         a.s[i[b]] = make_double2(sr, si);
+        if constexpr (sizeof(T) == 4) {
+            a.vf[i[b]] = make_float2(static_cast<float>(vi.x), static_cast<float>(vi.y));
+            a.sf[i[b]] = make_float2(static_cast<float>(sr), static_cast<float>(si));
+        }
         if (pe >= 0) {
             double const fp = sr - sp.x;
             a.f[pe] = fp;
@@ -225,31 +242,40 @@ __device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B]
 //   dP/d|V_j|  = (G Re e + B Im e) / |V_j|,   dQ/d|V_j| = (G Im e - B Re e) / |V_j|;
 // on the diagonal the reference's S-based forms.  A magnitude below 1e-6 is
 // replaced by 1 along the same direction, as the reference replaces |V| by 1.
-__device__ inline void jac_block_rect(bool diag, double2 yv, double2 vi, double2 vj, double2 si, double& pt,
-                                      double& pv, double& qt, double& qv) {
-    double const G = yv.x, B = yv.y;
-    double const ri = sqrt(vi.x * vi.x + vi.y * vi.y);
+template <typename R>
+__device__ inline R jac_div(R x, R d) {
+    if constexpr (sizeof(R) == 4) return x * __builtin_amdgcn_rcpf(d);  // 1 ulp; the entry is an FP32 factor input
+    else return x / d;
+}
+template <typename R, typename C>
+__device__ inline void jac_block_rect(bool diag, C yv, C vi, C vj, C si, R& pt, R& pv, R& qt, R& qv) {
+    R const G = yv.x, B = yv.y;
+    R const ri = sqrt(vi.x * vi.x + vi.y * vi.y);
     if (diag) {
-        double const Vi = ri < 1e-6 ? 1.0 : ri;
-        double const P = si.x, Q = si.y;
+        R const Vi = ri < R(1e-6) ? R(1) : ri;
+        R const P = si.x, Q = si.y;
         pt = -Q - B * Vi * Vi;
-        pv = (P / Vi) + G * Vi;
+        pv = jac_div(P, Vi) + G * Vi;
         qt = P - G * Vi * Vi;
-        qv = (Q / Vi) - B * Vi;
+        qv = jac_div(Q, Vi) - B * Vi;
         return;
     }
-    double rj = sqrt(vj.x * vj.x + vj.y * vj.y);
-    if (ri < 1e-6) vi = ri > 0.0 ? make_double2(vi.x / ri, vi.y / ri) : make_double2(1.0, 0.0);
-    if (rj < 1e-6) {
-        vj = rj > 0.0 ? make_double2(vj.x / rj, vj.y / rj) : make_double2(1.0, 0.0);
-        rj = 1.0;
+    R rj = sqrt(vj.x * vj.x + vj.y * vj.y);
+    if (ri < R(1e-6)) {
+        vi.x = ri > R(0) ? jac_div(vi.x, ri) : R(1);
+        vi.y = ri > R(0) ? jac_div(vi.y, ri) : R(0);
     }
-    double const re = vi.x * vj.x + vi.y * vj.y, im = vi.y * vj.x - vi.x * vj.y;
-    double const c = G * re + B * im, d = G * im - B * re;
+    if (rj < R(1e-6)) {
+        vj.x = rj > R(0) ? jac_div(vj.x, rj) : R(1);
+        vj.y = rj > R(0) ? jac_div(vj.y, rj) : R(0);
+        rj = R(1);
+    }
+    R const re = vi.x * vj.x + vi.y * vj.y, im = vi.y * vj.x - vi.x * vj.y;
+    R const c = G * re + B * im, d = G * im - B * re;
     pt = d;
     qt = -c;
-    pv = c / rj;
-    qv = d / rj;
+    pv = jac_div(c, rj);
+    qv = jac_div(d, rj);
 }
 
 // sin and cos of |x| <= 0.15 by their Taylor series (truncation below 1e-20)
@@ -294,7 +320,7 @@ template <typename T>
 struct TaskPre {
     int4 gf;                 // front record of the lane's front
     int4 pr[kPairChunks];    // pair list entries
-    double2 yv[kPairChunks]; // their per-unit Y
+    cplx_t<T> yv[kPairChunks]; // their per-unit Y
     int2 ex[kExtChunks];     // extend-add entries
     int rh;                  // right-hand-side LDS position
     T rv;                    // and its -F
@@ -320,7 +346,7 @@ __device__ inline TaskPre<T> load_pre(FusedArgs const& a, GridTask const& tk) {
 #pragma unroll
         for (int k = 0; k < kPairChunks; ++k) {
             r.pr[k] = ld_stream(a.pairs + tk.pair_beg + k * kWave + lane);
-            r.yv[k] = ld_stream(a.ypair + tk.pair_beg + k * kWave + lane);
+            r.yv[k] = ld_stream(static_cast<cplx_t<T> const*>(a.ypair) + tk.pair_beg + k * kWave + lane);
         }
 #pragma unroll
         for (int k = 0; k < kExtChunks; ++k) r.ex[k] = ld_stream(a.ext + tk.ext_beg + k * kWave + lane);
@@ -388,7 +414,19 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     int4 const* pr = pre.pr;
     int2 const* ex = pre.ex;
     int2 const vv = pre.vv;
-    double2 const* yv = pre.yv;
+    using C = cplx_t<T>;
+    C const* yv = pre.yv;
+    // the Jacobian inputs: V and S in the factor type
+    C const* vsrc;
+    C const* ssrc;
+    if constexpr (sizeof(T) == 4) {
+        vsrc = ma.vf;
+        ssrc = ma.sf;
+    } else {
+        vsrc = ma.v;
+        ssrc = ma.s;
+    }
+    C const* ypair = static_cast<C const*>(a.ypair);
     // gathers, unconditional: every list entry (the next task's and padding
     // included) holds valid indices; which entries are this task's is applied
     // where the values are used
@@ -401,14 +439,14 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     bool const vok = tk.vx_beg + lane < tk.vx_end && vv.y >= 0;
     // a lane whose value is not used reads lane 0's element instead, so its
     // load falls into a line the wavefront fetches anyway
-    double2 vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
+    C vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) {
         int const i0 = __builtin_amdgcn_readfirstlane(pr[k].x);
         bool const diag = pr[k].x == pr[k].y;
-        vi[k] = ma.v[pok[k] ? pr[k].x : i0];
-        vj[k] = ma.v[pok[k] && !diag ? pr[k].y : i0];
-        si[k] = ma.s[pok[k] && diag ? pr[k].x : i0];
+        vi[k] = vsrc[pok[k] ? pr[k].x : i0];
+        vj[k] = vsrc[pok[k] && !diag ? pr[k].y : i0];
+        si[k] = ssrc[pok[k] && diag ? pr[k].x : i0];
     }
     T exv[kExtChunks];
 #pragma unroll
@@ -426,21 +464,21 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
         for (int e = lane; e < (region + V - 1) / V; e += kWave) z[e] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     wave_sync();
-    auto put_pair = [&](bool ok, int4 const& p4, double2 y2, double2 pi2, double2 pj2, double2 s2) {
+    auto put_pair = [&](bool ok, int4 const& p4, C y2, C pi2, C pj2, C s2) {
         int const base = p4.w & 0xffff, mask = p4.w >> 16;
         if (!ok || !mask) return;
-        double pt, pv, qt, qv;
-        jac_block_rect(p4.x == p4.y, y2, pi2, pj2, s2, pt, pv, qt, qv);
-        if (mask & 1) pool[base] = static_cast<T>(pt);
-        if (mask & 2) pool[base + 1] = static_cast<T>(pv);
-        if (mask & 4) pool[base + ld] = static_cast<T>(qt);
-        if (mask & 8) pool[base + ld + 1] = static_cast<T>(qv);
+        T pt, pv, qt, qv;
+        jac_block_rect<T>(p4.x == p4.y, y2, pi2, pj2, s2, pt, pv, qt, qv);
+        if (mask & 1) pool[base] = pt;
+        if (mask & 2) pool[base + 1] = pv;
+        if (mask & 4) pool[base + ld] = qt;
+        if (mask & 8) pool[base + ld + 1] = qv;
     };
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) put_pair(pok[k], pr[k], yv[k], vi[k], vj[k], si[k]);
     for (int q = tk.pair_beg + kPairChunks * kWave + lane; q < tk.pair_end; q += kWave) {
         int4 const p4 = ld_stream(a.pairs + q);
-        put_pair(true, p4, ld_stream(a.ypair + q), ma.v[p4.x], ma.v[p4.y], ma.s[p4.x]);
+        put_pair(true, p4, ld_stream(ypair + q), vsrc[p4.x], vsrc[p4.y], ssrc[p4.x]);
     }
     if (rok) pool[pre.rh] = pre.rv;
     for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave)
@@ -641,10 +679,15 @@ __device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArg
     }
 }
 
-// Per-unit Y of every pair in pair order, once per solve.
-__global__ void __launch_bounds__(256) k_gather_ypair(int n, const int4* pairs, const double2* y, double2* ypair) {
+// Per-unit Y of every pair in pair order, in the factor type, once per solve.
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather_ypair(int n, const int4* pairs, const double2* y, cplx_t<T>* ypair) {
     int const k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) ypair[k] = y[pairs[k].z];
+    if (k < n) {
+        double2 const v = y[pairs[k].z];
+        ypair[k].x = static_cast<T>(v.x);
+        ypair[k].y = static_cast<T>(v.y);
+    }
 }
 
 // Progress of the wavefronts of a workgroup through a sweep: prog[w] is the

@@ -148,6 +148,10 @@ struct MisArgs {
     // factor type, in the order the factor tasks read it
     const int2* brhs;  // per bus: right-hand-side list positions of its P and Q equations (-1: none)
     void* frhs;        // one entry of the factor type per right-hand-side list entry
+    // grid schedule with FP32 factors: FP32 copies of V and S, the inputs of
+    // the FP32 Jacobian blocks, written by the mismatch pass
+    float2* vf;
+    float2* sf;
 };
 
 __device__ inline unsigned long long dbits(double x) { return __double_as_longlong(x); }
