@@ -558,10 +558,9 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
             w[t] -= (F[t * ld + k] * rp) * w[k];
         }
         wave_sync();
-        if (k < p && t == k)
-            for (int i = k + 1; i < m; ++i) F[i * ld + k] *= rp;
     }
-    wave_sync();
+    // (the multipliers below the diagonal are not scaled: the forward
+    // elimination above used them unscaled, and L is never stored)
     if (t < m) {
         T* Gm = front + pre.gf.x;
         for (int r = 0; r < m; ++r)
