@@ -245,13 +245,12 @@ struct DevFronts {
     DBuf<int4> fz_pairs;
     DBuf<double2> fz_ypair;  // Y value of each pair, refreshed at every solve
     long long fz_npairs = 0;
-    // -F of each right-hand-side entry (factor type), written by every mismatch
-    // pass; host: the equation of each entry, read once by the plan
-    DBuf<double> fz_frhs;
+    // -F of each Newton equation (factor type), written by every mismatch
+    // pass; host: the equation of each right-hand-side entry, read once by the plan
+    DBuf<double> fz_fe;
     std::vector<int> fz_rhs_eq;
     DBuf<int2> fz_rhs, fz_ext, fz_vx, fz_sol;
     long long fz_list_bytes = 0;
-    std::vector<double> fz_fixed, fz_welems;  // per instance, per Newton iteration (see build_grid)
     std::vector<long long> fz_work;  // front slots + factor entries of fz_order's networks (descending)
 
     BigMaps big() const { return BigMaps{bigbase.p, arow.p, crow.p, crow_ci.p}; }
@@ -474,28 +473,6 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
                 throw Error(GAPHIP_E_ARG, "grid task dependencies out of level order");
         }
     }
-    // algorithmic bytes per Newton iteration of each instance (factor + backward):
-    // fz_fixed[g] bytes of indices and FP64 inputs, fz_welems[g] entries of the
-    // factor precision
-    d.fz_fixed.assign(G, 0.0);
-    d.fz_welems.assign(G, 0.0);
-    for (size_t k = 0; k < tasks.size(); ++k) {
-        GridTask const& tk = tasks[k];
-        int const g = meta[tfront[tk.front_beg]].inst;
-        d.fz_fixed[g] += sizeof(GridTask) * 2.0 + 16.0 * (tk.pair_end - tk.pair_beg) + 4.0 * (tk.rhs_end - tk.rhs_beg) +
-                         8.0 * (tk.ext_end - tk.ext_beg) + 8.0 * (tk.vx_end - tk.vx_beg) + 16.0 * (tk.sol_end - tk.sol_beg);
-        // -F of each right-hand-side entry: written by the mismatch pass, read by the task
-        d.fz_welems[g] += 2.0 * (tk.rhs_end - tk.rhs_beg) + (tk.ext_end - tk.ext_beg) + (tk.vx_end - tk.vx_beg);
-        // per pair, in the factor type: its Y, and V_i with V_j or S_i (the Jacobian block's inputs)
-        d.fz_welems[g] += 6.0 * (tk.pair_end - tk.pair_beg);
-        for (int q = tk.front_beg; q < tk.front_beg + (tk.cls_cnt & 0xff); ++q) {
-            int const n = tfront[q];
-            double const m = meta[n].m, pp = meta[n].p;
-            d.fz_fixed[g] += 2.0 * sizeof(FrontMeta) + 8.0 * pp;  // metadata twice, solution writes
-            // stored U rows and Schur complement, w; then U rows and w read back
-            d.fz_welems[g] += (pp * m - pp * (pp - 1) / 2) + (m - pp) * (m - pp) + m + pp * m + pp;
-        }
-    }
     // bounds of every gathered index and LDS position
     for (auto const& pr : pairs)
         if (pr.x < 0 || pr.y < 0 || pr.z < 0 || (pr.w & 0xffff) + d.fz_ld4 + 1 >= (1 << 16))
@@ -545,8 +522,8 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     d.fz_pairs.upload(pairs, s);
     d.fz_npairs = static_cast<long long>(pairs.size());
     d.fz_ypair.alloc(d.fz_npairs);
-    d.fz_frhs.alloc(rhs.size());  // doubles: room for either factor type
-    HIPCHK(hipMemsetAsync(d.fz_frhs.p, 0, sizeof(double) * rhs.size(), s));
+    d.fz_fe.alloc(rhs.size());  // one per Newton equation; doubles: room for either factor type
+    HIPCHK(hipMemsetAsync(d.fz_fe.p, 0, sizeof(double) * rhs.size(), s));
     d.fz_rhs.upload(rhs, s);
     d.fz_ext.upload(ext, s);
     d.fz_vx.upload(vx, s);
@@ -1014,7 +991,6 @@ struct gaphip_plan {
     gaphip::DBuf<int> br_f, br_t, br_inst;  // in-service branch endpoints (global bus) or -1
     gaphip::DBuf<double> flow_zb, flow_ib, flow_out;  // branch flows (per instance bases, results)
     gaphip::DBuf<int> bus_type_d;
-    gaphip::DBuf<int2> brhs;  // grid schedule: right-hand-side list positions of each bus's equations
     gaphip::DBuf<double> adj_lossb, adj_gp, adj_gq, adj_gr, adj_gx, adj_gvm, adj_gva;
     gaphip::DBuf<double2> v, vpol, s, sspec, y_pu, y_given;
     gaphip::DBuf<float2> vf, sf;  // grid schedule with FP32 factors: V and S in FP32 (allocated on first use)
@@ -1321,23 +1297,15 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     P.var_off_d.upload(P.var_off, s);
     HIPCHK(hipStreamSynchronize(s));
     if (P.fr.fz_ok) {
-        // the grid mismatch pass writes each -F where the factor tasks read
-        // it: per bus the right-hand-side list positions of its equations
-        std::vector<int> eqpos(std::max<long long>(P.n_var, 1), -1);
-        auto const& req = P.fr.fz_rhs_eq;
-        for (size_t q = 0; q < req.size(); ++q) {
-            int const e = req[q];
-            if (e < 0 || e >= P.n_var || eqpos[e] >= 0) throw Error(GAPHIP_E_ARG, "right-hand-side entry out of range");
-            eqpos[e] = static_cast<int>(q);
+        // every Newton equation is the pivot of exactly one right-hand-side
+        // entry (the factor tasks gather its -F by equation index)
+        std::vector<char> seen(std::max<long long>(P.n_var, 1), 0);
+        for (int const e : P.fr.fz_rhs_eq) {
+            if (e < 0 || e >= P.n_var || seen[e]) throw Error(GAPHIP_E_ARG, "right-hand-side entry out of range");
+            seen[e] = 1;
         }
-        std::vector<int2> brhs(std::max<long long>(P.n_bus, 1), make_int2(-1, -1));
-        for (long long i = 0; i < P.n_bus; ++i) {
-            brhs[i] = make_int2(peq[i] >= 0 ? eqpos[peq[i]] : -1, qeq[i] >= 0 ? eqpos[qeq[i]] : -1);
-            if ((peq[i] >= 0 && brhs[i].x < 0) || (qeq[i] >= 0 && brhs[i].y < 0))
-                throw Error(GAPHIP_E_ARG, "equation without a pivot in the grid schedule");
-        }
-        P.brhs.upload(brhs, s);
-        HIPCHK(hipStreamSynchronize(s));
+        for (long long e = 0; e < P.n_var; ++e)
+            if (!seen[e]) throw Error(GAPHIP_E_ARG, "equation without a pivot in the grid schedule");
         std::vector<int>().swap(P.fr.fz_rhs_eq);
     }
 
@@ -1644,10 +1612,10 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
             P.sf.alloc(N);
         }
         FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, P.var_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
-                     d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_frhs.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
+                     d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_fe.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
                      P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, P.dx.p};
         MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
-                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p, P.brhs.p, d.fz_frhs.p, P.vf.p, P.sf.p};
+                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p, d.fz_fe.p, P.vf.p, P.sf.p};
         UpdArgs ua{static_cast<int>(N), P.bus_inst.p, P.done.p, P.lu_fail.p, P.ang.p, P.mag.p, P.dx.p, P.v.p};
         FrontArgs fra = d.args(nullptr, P.lu_fail.p);
         gaphip_tuning const& tu = P.tuning;
@@ -1676,8 +1644,16 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
     HIPCHK(hipStreamSynchronize(s));
     if (P.prof.on) {
         P.prof.collect();
-        // algorithmic bytes of the launch: per network, every full Newton
-        // iteration plus the final mismatch-only pass of a converged network
+        // algorithmic bytes of the launch, per network from its own sizes
+        // (N buses, Z Y-bus entries, V Newton variables, J Jacobian entries,
+        // L = nnz(L+U) of its fronts; w bytes per factor entry):
+        //   SpMV + mismatch + max   20 Z + 56 N + 8 V
+        //   Jacobian fill           20 Z + 44 N + w J
+        //   numeric factorisation   (w + 4) J + w L
+        //   triangular solves       (w + 4) L + 16 V
+        //   update                  8 V + 32 N
+        // for every full Newton iteration, plus the final mismatch-only pass
+        // of a converged network
         std::vector<int> its(G), done(G);
         HIPCHK(hipMemcpy(its.data(), P.iters.p, sizeof(int) * G, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(done.data(), P.done.p, sizeof(int) * G, hipMemcpyDeviceToHost));
@@ -1686,12 +1662,12 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
         for (int g = 0; g < G; ++g) {
             double const n = P.bus_off[g + 1] - P.bus_off[g], z = P.y_off[g + 1] - P.y_off[g];
             double const v = P.var_off[g + 1] - P.var_off[g];
+            double const J = static_cast<double>(P.fr.inst_arrow[g]), L = static_cast<double>(P.fr.inst_factor[g]);
             double const mis = 20.0 * z + 56.0 * n + 8.0 * v;
-            // the grid mismatch pass with FP32 factors also writes V and S in FP32
-            double const mis_f32 = sizeof(T) == 4 ? 16.0 * n : 0.0;
-            double const full = mis + mis_f32 + P.fr.fz_fixed[g] + w * P.fr.fz_welems[g] + (8.0 * v + 32.0 * n);
+            double const full = mis + (20.0 * z + 44.0 * n + w * J) + ((w + 4.0) * J + w * L) +
+                                ((w + 4.0) * L + 16.0 * v) + (8.0 * v + 32.0 * n);
             int const newton = done[g] ? its[g] - 1 : max_it;
-            total += newton * full + (done[g] ? mis + mis_f32 : 0.0);
+            total += newton * full + (done[g] ? mis : 0.0);
         }
         P.prof.bytes[K_FUSED] += total;
     }

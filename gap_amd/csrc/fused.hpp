@@ -56,7 +56,7 @@ struct FusedArgs {
     const int4* gfront;   // per front: (storage offset, vector offset, m | p << 16, first slot)
     const int4* pairs;    // (i, j, Y slot, LDS position | entry mask << 16)
     const void* ypair;    // per-unit Y value of each pair (the factor type's complex), in pair order (gathered once per solve)
-    const void* frhs;     // per right-hand-side entry: -F (factor type), written by the mismatch pass
+    const void* fe;       // per Newton equation: -F (factor type), written by the mismatch pass
     const int2* rhs;      // (equation index, LDS position)
     const int2* ext;      // (front storage index, LDS position)
     const int2* vx;       // (vector index, LDS position)
@@ -158,13 +158,14 @@ static_assert(kListPad >= kPairChunks * kWave && kListPad >= kExtChunks * kWave,
 
 // S_i = V_i conj((Y V)_i) and the P / Q mismatch rows of buses i[0 .. B-1]
 // (those past the first only where use_rest); returns their max |F|.  Each
-// equation's -F is also written, in the factor type, where the factor tasks
-// read it (their right-hand-side lists, contiguous per task).  The B rows'
+// equation's -F is written in the factor type, by equation index (the factor
+// tasks gather it); the FP64 F only for the network's first equation r0,
+// the one row the convergence test reads back.  The B rows'
 // loads are issued together, four entries per row and step; the sums keep the
 // row order.  B = 2 pays with FP64 factors; with FP32 factors its registers
 // would spill (the kernel is held to 128).
 template <typename T, int B>
-__device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B], bool use_rest) {
+__device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B], bool use_rest, int r0) {
 #pragma clang fp contract(off)
     double cr[B], cim[B];
     int qb[B], qe[B];
@@ -204,14 +205,13 @@ __device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B]
                     cim[b] += yv[b][u].x * vv[b][u].y + yv[b][u].y * vv[b][u].x;
                 }
     }
-    T* fr = static_cast<T*>(a.frhs);
+    T* fe = static_cast<T*>(a.fe);
     double m = 0.0;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         double2 const vi = a.v[i[b]];
         double2 const sp = a.sspec[i[b]];
         int const pe = a.peq[i[b]], qeq = a.qeq[i[b]];
-        int2 const rp = a.brhs[i[b]];
         if (b > 0 && !use_rest) break;
         double const sr = vi.x * cr[b] + vi.y * cim[b];
         double const si = vi.y * cr[b] - vi.x * cim[b];
@@ -222,14 +222,14 @@ __device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B]
         }
         if (pe >= 0) {
             double const fp = sr - sp.x;
-            a.f[pe] = fp;
-            fr[rp.x] = static_cast<T>(-1.0 * fp);
+            if (pe == r0) a.f[pe] = fp;
+            fe[pe] = static_cast<T>(-1.0 * fp);
             m = fmax(m, fabs(fp));
         }
         if (qeq >= 0) {
             double const fq = si - sp.y;
-            a.f[qeq] = fq;
-            fr[rp.y] = static_cast<T>(-1.0 * fq);
+            if (qeq == r0) a.f[qeq] = fq;
+            fe[qeq] = static_cast<T>(-1.0 * fq);
             m = fmax(m, fabs(fq));
         }
     }
@@ -322,16 +322,10 @@ struct TaskPre {
     int4 pr[kPairChunks];    // pair list entries
     cplx_t<T> yv[kPairChunks]; // their per-unit Y
     int2 ex[kExtChunks];     // extend-add entries
-    int rh;                  // right-hand-side LDS position
-    T rv;                    // and its -F
+    int2 rq;                 // right-hand-side entry (equation, LDS position)
     int2 vv;                 // child-vector entry
     int2 so;                 // (backward) solution entries
 };
-
-// LDS position of right-hand-side entry q (its equation index is not needed)
-__device__ inline int rhs_pos(FusedArgs const& a, int q) {
-    return __builtin_nontemporal_load(reinterpret_cast<const int*>(a.rhs + q) + 1);
-}
 
 // Unconditional: entries past the task's lists are the next task's or padding.
 template <typename T, bool BWD>
@@ -350,8 +344,7 @@ __device__ inline TaskPre<T> load_pre(FusedArgs const& a, GridTask const& tk) {
         }
 #pragma unroll
         for (int k = 0; k < kExtChunks; ++k) r.ex[k] = ld_stream(a.ext + tk.ext_beg + k * kWave + lane);
-        r.rh = rhs_pos(a, tk.rhs_beg + lane);
-        r.rv = __builtin_nontemporal_load(static_cast<T const*>(a.frhs) + tk.rhs_beg + lane);
+        r.rq = ld_stream(a.rhs + tk.rhs_beg + lane);
         r.vv = ld_stream(a.vx + tk.vx_beg + lane);
     }
     return r;
@@ -435,7 +428,7 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     for (int k = 0; k < kPairChunks; ++k) pok[k] = tk.pair_beg + k * kWave + lane < tk.pair_end;
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k) eok[k] = tk.ext_beg + k * kWave + lane < tk.ext_end && ex[k].y >= 0;
-    bool const rok = tk.rhs_beg + lane < tk.rhs_end && pre.rh >= 0;
+    bool const rok = tk.rhs_beg + lane < tk.rhs_end && pre.rq.y >= 0;
     bool const vok = tk.vx_beg + lane < tk.vx_end && vv.y >= 0;
     // a lane whose value is not used reads lane 0's element instead, so its
     // load falls into a line the wavefront fetches anyway
@@ -453,6 +446,8 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     for (int k = 0; k < kExtChunks; ++k)
         exv[k] = ld_stream(front + (eok[k] ? ex[k].x : __builtin_amdgcn_readfirstlane(ex[k].x)));
     T const vxv = vec[vok ? vv.x : __builtin_amdgcn_readfirstlane(vv.x)];
+    T const* fe = static_cast<T const*>(a.fe);
+    T const rv = fe[rok ? pre.rq.x : __builtin_amdgcn_readfirstlane(pre.rq.x)];
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
     wave_sync();  // the previous task of this wavefront has finished with the region
@@ -480,9 +475,11 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
         int4 const p4 = ld_stream(a.pairs + q);
         put_pair(true, p4, ld_stream(ypair + q), vsrc[p4.x], vsrc[p4.y], ssrc[p4.x]);
     }
-    if (rok) pool[pre.rh] = pre.rv;
-    for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave)
-        pool[rhs_pos(a, q)] = static_cast<T const*>(a.frhs)[q];
+    if (rok) pool[pre.rq.y] = rv;
+    for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave) {
+        int2 const e = ld_stream(a.rhs + q);
+        pool[e.y] = fe[e.x];
+    }
     wave_sync();
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k)
@@ -789,6 +786,7 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1)
     int const g = a.order[a.order_off + blockIdx.x];
     int const b0 = a.bus_off[g], b1 = a.bus_off[g + 1];
     int const t_lo = lvl_ptr[a.lvl_beg[g]], t_hi = lvl_ptr[a.lvl_beg[g + 1]];  // the network's tasks
+    int const r0 = a.var_off[g], r1 = a.var_off[g + 1];                       // and its Newton equations
     int const tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     T* pool = reinterpret_cast<T*>(grid_smem) + wave * a.pool;
     for (int it = 0; it < a.max_it; ++it) {
@@ -805,10 +803,10 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1)
             int ib[MB];
 #pragma unroll
             for (int b = 0; b < MB; ++b) ib[b] = min(i + b * NT, b1 - 1);
-            m = fmax(m, grid_mismatch_buses<T, MB>(ma, ib, i + (MB - 1) * NT < b1));
+            m = fmax(m, grid_mismatch_buses<T, MB>(ma, ib, i + (MB - 1) * NT < b1, r0));
         }
         m = block_max<NW>(m, red);  // its barriers also publish S and F
-        if (int const r0 = a.var_off[g]; r0 < a.var_off[g + 1]) m = reference_max(m, ma.f[r0]);
+        if (r0 < r1) m = reference_max(m, ma.f[r0]);
         if (tid == 0) {
             a.maxf[static_cast<long long>(it) * a.n_inst + g] = dbits(m);
             a.final_mis[g] = m;

@@ -144,10 +144,10 @@ struct MisArgs {
     double2* vpol;  // (|V|, arg V) for the Jacobian
     double* f;
     unsigned long long* maxf;  // this iteration's row of per-instance maxima
-    // grid schedule only: the mismatch pass also writes each -F, in the
-    // factor type, in the order the factor tasks read it
-    const int2* brhs;  // per bus: right-hand-side list positions of its P and Q equations (-1: none)
-    void* frhs;        // one entry of the factor type per right-hand-side list entry
+    // grid schedule only: the mismatch pass also writes each -F in the factor
+    // type, per Newton equation (the factor tasks gather it), and the FP64 F
+    // of each network's first equation only (the convergence test's first row)
+    void* fe;
     // grid schedule with FP32 factors: FP32 copies of V and S, the inputs of
     // the FP32 Jacobian blocks, written by the mismatch pass
     float2* vf;
