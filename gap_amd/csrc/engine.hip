@@ -241,7 +241,7 @@ struct DevFronts {
     int fz_pool = 0, fz_ld4 = 1;
     DBuf<int> fz_order, fz_lvl_beg, fz_lvl_ptr;
     DBuf<GridTask> fz_tasks;
-    DBuf<int4> fz_gfront;  // per front: storage offset, vector offset, m | p << 16, first slot
+    DBuf<int4> fz_gfront;  // per front: storage offset, vector offset, m | p << 16, list position of its first pivot
     DBuf<int4> fz_pairs;
     DBuf<double2> fz_ypair;  // Y value of each pair, refreshed at every solve
     long long fz_npairs = 0;
@@ -249,6 +249,11 @@ struct DevFronts {
     // pass; host: the equation of each right-hand-side entry, read once by the plan
     DBuf<double> fz_fe;
     std::vector<int> fz_rhs_eq;
+    // solution of each pivot, in right-hand-side list order (factor type;
+    // doubles: room for either), written by the backward tasks; host: the
+    // list position of each Newton variable, read once by the plan
+    DBuf<double> fz_xs;
+    std::vector<int> fz_varpos;
     DBuf<int2> fz_rhs, fz_ext, fz_vx, fz_sol;
     long long fz_list_bytes = 0;
     std::vector<long long> fz_work;  // front slots + factor entries of fz_order's networks (descending)
@@ -293,6 +298,7 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     int const nf = d.n_fronts, K = kNumClasses;
     constexpr long long kIntMax = 0x7fffffffLL;
     d.fz_rhs_eq.clear();
+    d.fz_varpos.clear();
     d.fz_ok = nf > 0 && d.max_front <= kClassMax[kGridClasses - 1] && d.front_entries < kIntMax &&
               d.vec_entries < kIntMax && n_index < kIntMax && n_values < kIntMax;
     if (!d.fz_ok) return;
@@ -389,6 +395,8 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     std::vector<GridTask> tasks;
     std::vector<int4> pairs;
     std::vector<int2> rhs, ext, vx, sol;
+    std::vector<int> fpos(nf, -1);               // per front: list position of its first pivot
+    std::vector<int> varpos(n_index, -1);        // per Newton variable: list position of its pivot
     int pool = 1;
     for (auto const& lv : per) {
         for (int2 const& tr : lv) {
@@ -418,7 +426,14 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
                             throw Error(GAPHIP_E_ARG, "Jacobian pair outside its front");
                     pairs.push_back(make_int4(it.it_i[t], it.it_j[t], it.it_y[t], (tile + r0 * ld + c0) | (mask << 16)));
                 }
-                for (int t = 0; t < fm.p; ++t) rhs.push_back(make_int2(slot_row[fm.slot_beg + t], wpos + t));
+                fpos[n] = static_cast<int>(rhs.size());
+                for (int t = 0; t < fm.p; ++t) {
+                    int const v = slot_col[fm.slot_beg + t];
+                    if (v < 0 || v >= n_index || varpos[v] >= 0) throw Error(GAPHIP_E_ARG, "pivot variable out of range");
+                    varpos[v] = static_cast<int>(rhs.size());
+                    rhs.push_back(make_int2(slot_row[fm.slot_beg + t], wpos + t));
+                }
+                // (variable index here; its pivot's list position once every task is built)
                 for (int t = fm.p; t < fm.m; ++t) sol.push_back(make_int2(slot_col[fm.slot_beg + t], wpos + t));
                 for (int ci = fm.child_beg; ci < fm.child_end; ++ci) {
                     ChildInfo const& ch = cinfo[ci];
@@ -443,6 +458,12 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
             tasks.push_back(tk);
         }
         lvl_ptr.push_back(static_cast<int>(tasks.size()));
+    }
+    // the backward tasks gather the solution of each structure slot at its
+    // pivot's list position
+    for (auto& e : sol) {
+        if (e.x < 0 || e.x >= n_index || varpos[e.x] < 0) throw Error(GAPHIP_E_ARG, "structure variable without a pivot");
+        e.x = varpos[e.x];
     }
     // Dependencies between the tasks of a network: the task holding the last
     // child (factorisation) and the first parent (backward substitution) of
@@ -482,12 +503,13 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
             if (e.x < 0 || e.x >= src_max || e.y < 0 || e.y >= pool) throw Error(GAPHIP_E_ARG, std::string(what) + " out of range");
     };
     check2(rhs, n_index, "rhs entry");
-    check2(sol, n_index, "solution entry");
+    check2(sol, static_cast<long long>(rhs.size()), "solution entry");
     check2(ext, d.front_entries, "extend-add entry");
     check2(vx, d.vec_entries, "vector entry");
     d.fz_pool = (pool + 3) / 4 * 4;  // whole 16-byte units per wavefront (the task clears its region with 16-byte stores)
     d.fz_rhs_eq.resize(rhs.size());
     for (size_t q = 0; q < rhs.size(); ++q) d.fz_rhs_eq[q] = rhs[q].x;
+    d.fz_varpos = std::move(varpos);
     // largest networks first: they bound the launch
     std::vector<int> order(G);
     for (int g = 0; g < G; ++g) order[g] = g;
@@ -505,7 +527,9 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
         if (fm.off < 0 || fm.off > kIntMax || fm.voff < 0 || fm.voff > kIntMax || fm.m < 0 || fm.m > 0x7fff ||
             fm.p < 0 || fm.p > fm.m)
             throw Error(GAPHIP_E_ARG, "front record out of range");
-        gfront[k] = make_int4(static_cast<int>(fm.off), static_cast<int>(fm.voff), fm.m | (fm.p << 16), fm.slot_beg);
+        if (fm.p > 0 && (fpos[n] < 0 || fpos[n] + fm.p > static_cast<long long>(rhs.size())))
+            throw Error(GAPHIP_E_ARG, "front pivots outside the right-hand-side list");
+        gfront[k] = make_int4(static_cast<int>(fm.off), static_cast<int>(fm.voff), fm.m | (fm.p << 16), std::max(fpos[n], 0));
     }
     // padding read by the grid kernel's unconditional loads (never executed)
     tasks.insert(tasks.end(), kTaskPad, GridTask{});
@@ -524,6 +548,8 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     d.fz_ypair.alloc(d.fz_npairs);
     d.fz_fe.alloc(rhs.size());  // one per Newton equation; doubles: room for either factor type
     HIPCHK(hipMemsetAsync(d.fz_fe.p, 0, sizeof(double) * rhs.size(), s));
+    d.fz_xs.alloc(rhs.size() + kVecPad);
+    HIPCHK(hipMemsetAsync(d.fz_xs.p, 0, sizeof(double) * (rhs.size() + kVecPad), s));
     d.fz_rhs.upload(rhs, s);
     d.fz_ext.upload(ext, s);
     d.fz_vx.upload(vx, s);
@@ -1000,6 +1026,7 @@ struct gaphip_plan {
     gaphip::DBuf<float> front32, vec32;
     gaphip::DevFronts fr;
     gaphip::DBuf<int> bus_off_d;  // per-instance bus offsets for the grid schedule
+    gaphip::DBuf<int2> bpos;      // grid schedule: per bus, list positions of its angle / magnitude variables (-1: none)
     gaphip::DBuf<int> var_off_d;  // per-instance first Newton equation (n_inst + 1)
     int schedule = GAPHIP_SCHED_AUTO;
     gaphip_tuning tuning{2, 8, -1, 1, gaphip::kBigOuter, gaphip::kTwoLevelMin};
@@ -1307,6 +1334,18 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
         for (long long e = 0; e < P.n_var; ++e)
             if (!seen[e]) throw Error(GAPHIP_E_ARG, "equation without a pivot in the grid schedule");
         std::vector<int>().swap(P.fr.fz_rhs_eq);
+        // the grid update gathers each bus's corrections at its variables' list positions
+        auto const& vp = P.fr.fz_varpos;
+        std::vector<int2> bpos(std::max<long long>(P.n_bus, 1), make_int2(-1, -1));
+        for (long long i = 0; i < P.n_bus; ++i) {
+            int const a = ang[i] >= 0 ? vp[ang[i]] : -1, m = mag[i] >= 0 ? vp[mag[i]] : -1;
+            if ((ang[i] >= 0 && a < 0) || (mag[i] >= 0 && m < 0))
+                throw Error(GAPHIP_E_ARG, "variable without a pivot in the grid schedule");
+            bpos[i] = make_int2(a, m);
+        }
+        P.bpos.upload(bpos, s);
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<int>().swap(P.fr.fz_varpos);
     }
 
     // per-bus / per-instance state and work arrays
@@ -1613,10 +1652,11 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
         }
         FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, P.var_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
                      d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_fe.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
-                     P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, P.dx.p};
+                     P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, d.fz_xs.p};
         MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
                    P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p, d.fz_fe.p, P.vf.p, P.sf.p};
-        UpdArgs ua{static_cast<int>(N), P.bus_inst.p, P.done.p, P.lu_fail.p, P.ang.p, P.mag.p, P.dx.p, P.v.p};
+        UpdArgs ua{static_cast<int>(N), P.bus_inst.p, P.done.p, P.lu_fail.p, P.ang.p, P.mag.p, P.dx.p, P.v.p,
+                   P.bpos.p, d.fz_xs.p};
         FrontArgs fra = d.args(nullptr, P.lu_fail.p);
         gaphip_tuning const& tu = P.tuning;
         int nbig = 0;

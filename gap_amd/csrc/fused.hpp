@@ -20,7 +20,9 @@
 //   ext   : child Schur-complement entries, added into the front (LDS atomics,
 //           list order = child order, so every solve sums in the same order);
 //   vx    : child right-hand-side entries, added likewise;
-//   sol   : (backward) solution values of the structure slots.
+//   sol   : (backward) solution values of the structure slots, gathered at
+//           their pivots' list positions; each task writes its own pivots'
+//           solution at their list positions, contiguous per task.
 // Inside a task only wavefront-level synchronisation is needed.  Between tasks
 // of different wavefronts the order is kept by dependencies, not by a barrier
 // per level: a wavefront publishes in LDS the index of the next task it will
@@ -53,7 +55,7 @@ struct FusedArgs {
     const int* lvl_beg;   // per network: its levels are [lvl_beg[g], lvl_beg[g+1]) of lvl_ptr
     const int* lvl_ptr;   // per (network, level): tasks [lvl_ptr[e], lvl_ptr[e+1])
     const GridTask* tasks;
-    const int4* gfront;   // per front: (storage offset, vector offset, m | p << 16, first slot)
+    const int4* gfront;   // per front: (storage offset, vector offset, m | p << 16, list position of its first pivot)
     const int4* pairs;    // (i, j, Y slot, LDS position | entry mask << 16)
     const void* ypair;    // per-unit Y value of each pair (the factor type's complex), in pair order (gathered once per solve)
     const void* fe;       // per Newton equation: -F (factor type), written by the mismatch pass
@@ -68,7 +70,7 @@ struct FusedArgs {
     int* iters;
     int* lu_fail;
     double* final_mis;
-    double* dx;
+    void* xs;  // solution of each pivot (factor type), in right-hand-side list order
 };
 
 __device__ inline void wave_sync() {
@@ -289,10 +291,13 @@ __device__ inline void sincos_small(double x, double& sn, double& cs) {
 
 // Damped, clamped polar update: rotate V by the angle step, then rescale to the
 // clamped new magnitude.
+template <typename T>
 __device__ inline void grid_update_bus(UpdArgs const& a, int i, bool zero) {
     double2 v = a.v[i];
-    int av = a.ang[i], mv = a.mag[i];
-    double const dxa = a.dx[av < 0 ? 0 : av], dxm = a.dx[mv < 0 ? 0 : mv];  // unconditional loads
+    int2 const bp = a.bpos[i];
+    int const av = bp.x, mv = bp.y;
+    T const* xs = static_cast<T const*>(a.xs);
+    double const dxa = static_cast<double>(xs[av < 0 ? 0 : av]), dxm = static_cast<double>(xs[mv < 0 ? 0 : mv]);  // unconditional loads
     if (av >= 0) {
         double ac = 0.9 * (zero ? 0.0 : dxa);
         if (fabs(ac) > 0.15) ac = 0.15 * (ac > 0 ? 1.0 : -1.0);
@@ -569,7 +574,7 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
 // Backward substitution of the pivot block of up to kWave / L fronts.
 template <typename T, int MM, int L>
 __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs const& a, FrontArgs const& fa, GridTask const& tk, TaskPre<T> const& pre,
-                                   T const* front, T const* vec, double* sol, T* pool, int ld) {
+                                   T const* front, T const* vec, T* xsol, T* pool, int ld) {
     constexpr int G = kWave / L, CH = MM < 16 ? MM : 16;
     int const lane = lane_id(), gi = lane / L, t = lane % L;
     int const rows = ld - 1, wbase = G * rows * ld;
@@ -580,16 +585,15 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
     // unconditional loads: the front record of an inactive lane is another
     // front's or padding
     // (a lane whose value is not used reads lane 0's element: same line)
-    T const sv = static_cast<T>(sol[sok ? so.x : __builtin_amdgcn_readfirstlane(so.x)]);
+    T const sv = xsol[sok ? so.x : __builtin_amdgcn_readfirstlane(so.x)];
     T const wv = vec[t < p ? pre.gf.y + t : __builtin_amdgcn_readfirstlane(pre.gf.y)];
-    int const colt = fa.slot_col[t < p ? pre.gf.w + t : __builtin_amdgcn_readfirstlane(pre.gf.w)];
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
     wave_sync();  // the previous task of this wavefront has finished with the region
     if (sok) pool[so.y] = sv;
     for (int q = tk.sol_beg + kWave + lane; q < tk.sol_end; q += kWave) {
         int2 const e = ld_stream(a.sol + q);
-        pool[e.y] = static_cast<T>(sol[e.x]);
+        pool[e.y] = xsol[e.x];
     }
     if (t < p) w[t] = wv;
     T xme = T(0);  // this lane's solution value (t < p)
@@ -626,7 +630,7 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
                 }
             }
         }
-        if (t < p) sol[colt] = static_cast<double>(xme);
+        if (t < p) xsol[pre.gf.w + t] = xme;
         return;
     }
     load_rows<T, MM, L, CH>(F, ld, front + pre.gf.x, m, p, t, t < m);
@@ -646,7 +650,7 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
         }
         wave_sync();
     }
-    if (t < p) sol[colt] = static_cast<double>(xme);
+    if (t < p) xsol[pre.gf.w + t] = xme;
 }
 
 template <typename T, bool BWD>
@@ -654,23 +658,23 @@ __device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArg
                                  TaskPre<T> const& pre, T* front, T* vec, T* pool) {
     switch (tk.cls_cnt >> 8) {
         case 0:
-            if constexpr (BWD) grid_backward_task<T, 4, 4>(a, fa, tk, pre, front, vec, a.dx, pool, 5);
+            if constexpr (BWD) grid_backward_task<T, 4, 4>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 5);
             else grid_factor_task<T, 4, 4>(a, fa, ma, tk, pre, g, front, vec, pool, 5);
             break;
         case 1:
-            if constexpr (BWD) grid_backward_task<T, 8, 8>(a, fa, tk, pre, front, vec, a.dx, pool, 9);
+            if constexpr (BWD) grid_backward_task<T, 8, 8>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 9);
             else grid_factor_task<T, 8, 8>(a, fa, ma, tk, pre, g, front, vec, pool, 9);
             break;
         case 2:
-            if constexpr (BWD) grid_backward_task<T, 16, 16>(a, fa, tk, pre, front, vec, a.dx, pool, 17);
+            if constexpr (BWD) grid_backward_task<T, 16, 16>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 17);
             else grid_factor_task<T, 16, 16>(a, fa, ma, tk, pre, g, front, vec, pool, 17);
             break;
         case 3:
-            if constexpr (BWD) grid_backward_task<T, 32, 32>(a, fa, tk, pre, front, vec, a.dx, pool, 33);
+            if constexpr (BWD) grid_backward_task<T, 32, 32>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 33);
             else grid_factor_task<T, 32, 32>(a, fa, ma, tk, pre, g, front, vec, pool, 33);
             break;
         default:
-            if constexpr (BWD) grid_backward_task<T, 64, 64>(a, fa, tk, pre, front, vec, a.dx, pool, a.ld4);
+            if constexpr (BWD) grid_backward_task<T, 64, 64>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, a.ld4);
             else grid_factor_task<T, 64, 64>(a, fa, ma, tk, pre, g, front, vec, pool, a.ld4);
     }
 }
@@ -824,7 +828,7 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1)
         run_sweep<T, true, NW>(a, fa, ma, tasks, g, wave, t_lo, t_hi, front, vec, pool, prog_b);
         __syncthreads();
         bool const zero = a.lu_fail[g] != 0;
-        for (int i = b0 + tid; i < b1; i += NT) grid_update_bus(ua, i, zero);
+        for (int i = b0 + tid; i < b1; i += NT) grid_update_bus<T>(ua, i, zero);
         __syncthreads();
     }
 }

@@ -1370,6 +1370,11 @@ struct UpdArgs {
     const int* mag;
     const double* dx;
     double2* v;
+    // grid schedule: per bus the list positions of its angle and magnitude
+    // variables in xs, the solution in the factor type in right-hand-side
+    // list order (-1: no such variable)
+    const int2* bpos;
+    const void* xs;
 };
 __device__ inline void update_bus(UpdArgs const& a, int i, bool zero) {
 #pragma clang fp contract(off)
