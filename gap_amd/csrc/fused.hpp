@@ -587,6 +587,19 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
     // (a lane whose value is not used reads lane 0's element: same line)
     T const sv = xsol[sok ? so.x : __builtin_amdgcn_readfirstlane(so.x)];
     T const wv = vec[t < p ? pre.gf.y + t : __builtin_amdgcn_readfirstlane(pre.gf.y)];
+    int const p_max = wave_group_max<L>(p);
+    // small fronts: the U rows by columns (lane t reads column t of each pivot
+    // row, consecutive along the row), transposed through the group's LDS tile
+    T ucol[MM <= 16 ? MM : 1];
+    if constexpr (MM <= 16) {
+        int const g0 = __builtin_amdgcn_readfirstlane(pre.gf.x);
+#pragma unroll
+        for (int r = 0; r < MM; ++r) {
+            if (r >= p_max) continue;  // uniform
+            bool const use = r < p && t < m;
+            ucol[r] = ld_stream(front + (use ? pre.gf.x + r * m + t : g0));
+        }
+    }
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
     wave_sync();  // the previous task of this wavefront has finished with the region
@@ -600,16 +613,13 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
     if constexpr (MM <= 16) {
         // lane t < p holds row t of U; x of the structure slots and the steps'
         // x_k come from the owning lanes by shuffles
-        T urow[MM];
-        int const g0 = __builtin_amdgcn_readfirstlane(pre.gf.x);
-        int const rbeg = pre.gf.x + t * m;
 #pragma unroll
-        for (int q = 0; q < MM; ++q) {
-            bool const use = t < p && q < m;
-            T const x = ld_stream(front + (use ? rbeg + q : g0));
-            urow[q] = use ? x : T(0);
-        }
+        for (int r = 0; r < MM; ++r)
+            if (r < p_max && r < p && t < m) F[r * ld + t] = ucol[r];
         wave_sync();
+        T urow[MM];
+#pragma unroll
+        for (int q = 0; q < MM; ++q) urow[q] = t < p && q < m ? F[t * ld + q] : T(0);
         T acc = t < m ? w[t] : T(0);
         T xs[MM];
 #pragma unroll
@@ -619,7 +629,6 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
             for (int q = 0; q < MM; ++q)
                 if (q >= p && q < m) acc -= urow[q] * xs[q];
         }
-        int const p_max = wave_group_max<L>(p);
 #pragma unroll
         for (int k = MM - 1; k >= 0; --k) {
             if (k < p_max) {
@@ -641,7 +650,6 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
         w[t] = acc;
     }
     wave_sync();
-    int const p_max = wave_group_max<L>(p);
     for (int k = p_max - 1; k >= 0; --k) {
         if (k < p) {
             T const xk = pivot_div(w[k], F[k * ld + k]);
