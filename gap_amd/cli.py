@@ -49,6 +49,9 @@ def main(argv=None) -> int:
     ap.add_argument("--repeat", type=int, default=1, help="time this many solves after the first")
     ap.add_argument("--graph", choices=["default", "on", "off"], default="default",
                     help="level schedule: replay the Newton step from a HIP graph, or launch its kernels one by one")
+    ap.add_argument("--grid-shape", metavar="W,B,WORK",
+                    help="grid schedule launch shape (the autotuner's knobs): wavefronts per workgroup, wavefronts "
+                         "per workgroup of big networks, front slots + factor entries from which a network is big")
     ap.add_argument("--probe-copy", action="store_true",
                     help="after the solves, copy a 512 MiB device buffer 3 times (calibrates HBM byte counters)")
     args = ap.parse_args(argv)
@@ -72,6 +75,9 @@ def main(argv=None) -> int:
     plan = capi.Plan(nets, args.precision, schedule=args.schedule)
     if args.graph != "default":
         plan.set_graph(args.graph == "on")
+    if args.grid_shape:
+        w, b, work = (int(x) for x in args.grid_shape.split(","))
+        plan.set_tuning(grid_waves=w, grid_big_waves=b, grid_big_work=work)
     t_plan = time.perf_counter() - t0
     t0 = time.perf_counter()
     res = plan.solve(args.tolerance, args.max_iter, base)
