@@ -194,6 +194,9 @@ constexpr int kNumClasses = 7;
 constexpr int kClassMax[kNumClasses - 1] = {4, 8, 16, 32, 64, 128};
 constexpr int kClassPerBlock[kNumClasses - 1] = {16, 8, 4, 2, 1, 1};  // fronts per workgroup
 constexpr int kGridClasses = 5;  // classes the grid schedule handles (fronts <= 64 slots)
+// fronts per grid task of each class (4, 8, 16 lanes per front; a whole
+// wavefront per front of up to 32 or 64 slots)
+constexpr int kGridPerTask[kGridClasses] = {16, 8, 4, 1, 1};
 inline int front_class(int m) {
     for (int c = 0; c < kNumClasses - 1; ++c)
         if (m <= kClassMax[c]) return c;
@@ -284,8 +287,8 @@ struct GridItems {
     std::vector<int> const& it_pos;
 };
 
-// Grid schedule of a packed plan: tasks (runs of at most 64 / L fronts of one
-// instance, level and class) and the flat assembly lists each task indexes.
+// Grid schedule of a packed plan: tasks (runs of at most kGridPerTask fronts of
+// one instance, level and class) and the flat assembly lists each task indexes.
 // Within a (level, class) bucket the fronts of one instance are contiguous
 // (the renumbering is stable), so a task is a contiguous id range.  Every
 // index and LDS position is checked here.
@@ -372,7 +375,7 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     for (size_t e = 0; e < per.size(); ++e) {
         auto& bc = byc[e];
         for (int c = kGridClasses - 1; c >= 0; --c) {
-            int const per_task = kWave / kClassMax[c];
+            int const per_task = kGridPerTask[c];
             auto const& F = bc[c];
             size_t i = 0;
             while (i < F.size()) {
@@ -400,7 +403,7 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     int pool = 1;
     for (auto const& lv : per) {
         for (int2 const& tr : lv) {
-            int const c = tr.y >> 8, cnt = tr.y & 0xff, L = kClassMax[c], Gc = kWave / L;
+            int const c = tr.y >> 8, cnt = tr.y & 0xff, L = kClassMax[c], Gc = kGridPerTask[c];
             int const ld = c == kGridClasses - 1 ? d.fz_ld4 : L + 1, rows = ld - 1;
             int const wbase = Gc * rows * ld, region = wbase + Gc * L;
             pool = std::max(pool, region);

@@ -10,8 +10,10 @@
 //   per-pair arithmetic uses the rectangular forms below.
 //
 // A level of the network's elimination tree is a list of tasks; a task is up to
-// 64 / L fronts of one size class handled by one wavefront (L lanes per front,
-// lane t owning column t, the fronts staged in the wavefront's LDS region).
+// 16, 8 or 4 fronts of at most 4, 8 or 16 slots (4, 8 or 16 lanes per front),
+// or one front of at most 32 or 64 slots (the whole wavefront), handled by
+// one wavefront (lane t of a front's lanes owning column t, the fronts staged
+// in the wavefront's LDS region).
 // Assembly is driven by flat per-task lists built once per plan, so all 64
 // lanes gather the task's inputs together (two dependent loads per entry):
 //   pairs : one per Jacobian 2x2 block (buses i, j, Y slot, LDS position and
@@ -127,6 +129,11 @@ __device__ inline int lane_id() {
     asm volatile("" : "+v"(l));
     return l & (kWave - 1);
 }
+
+// Lanes per front of each grid task class, as a shift: 4, 8 and 16 lanes
+// for fronts of up to 4, 8 and 16 slots (16, 8 and 4 fronts per task), a
+// whole wavefront for fronts of up to 32 and 64 slots (one front per task).
+__device__ inline int grid_lane_shift(int cls) { return cls < 3 ? 2 + cls : 6; }
 
 template <typename T>
 __device__ inline void lds_add(T* p, T v) {
@@ -280,13 +287,24 @@ __device__ inline void jac_block_rect(bool diag, C yv, C vi, C vj, C si, R& pt, 
     qv = jac_div(d, rj);
 }
 
+// An FP64 constant materialised in scalar registers where it is used.  FP64
+// operands other than a few inline values cannot be encoded in an
+// instruction; left to itself the compiler hoists them out of the Newton
+// loop into vector registers, where they stay live for the whole kernel.
+__device__ inline double sconst(double c) {
+    asm volatile("" : "+s"(c));
+    return c;
+}
+
 // sin and cos of |x| <= 0.15 by their Taylor series (truncation below 1e-20)
 __device__ inline void sincos_small(double x, double& sn, double& cs) {
     double const x2 = x * x;
-    sn = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880 +
-                                                                                     x2 * (-1.0 / 39916800))))));
-    cs = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24 + x2 * (-1.0 / 720 + x2 * (1.0 / 40320 + x2 * (-1.0 / 3628800 +
-                                                                                           x2 * (1.0 / 479001600))))));
+    sn = x * (1.0 + x2 * (sconst(-1.0 / 6) + x2 * (sconst(1.0 / 120) + x2 * (sconst(-1.0 / 5040) +
+                                                                           x2 * (sconst(1.0 / 362880) +
+                                                                                 x2 * sconst(-1.0 / 39916800))))));
+    cs = 1.0 + x2 * (-0.5 + x2 * (sconst(1.0 / 24) + x2 * (sconst(-1.0 / 720) + x2 * (sconst(1.0 / 40320) +
+                                                                                 x2 * (sconst(-1.0 / 3628800) +
+                                                                                       x2 * sconst(1.0 / 479001600))))));
 }
 
 // Damped, clamped polar update: rotate V by the angle step, then rescale to the
@@ -299,17 +317,19 @@ __device__ inline void grid_update_bus(UpdArgs const& a, int i, bool zero) {
     T const* xs = static_cast<T const*>(a.xs);
     double const dxa = static_cast<double>(xs[av < 0 ? 0 : av]), dxm = static_cast<double>(xs[mv < 0 ? 0 : mv]);  // unconditional loads
     if (av >= 0) {
-        double ac = 0.9 * (zero ? 0.0 : dxa);
-        if (fabs(ac) > 0.15) ac = 0.15 * (ac > 0 ? 1.0 : -1.0);
+        double ac = sconst(0.9) * (zero ? 0.0 : dxa);
+        double const amax = sconst(0.15);
+        if (fabs(ac) > amax) ac = amax * (ac > 0 ? 1.0 : -1.0);
         double sn, cs;
         sincos_small(ac, sn, cs);
         v = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
     }
     if (mv >= 0) {
         double const old = sqrt(v.x * v.x + v.y * v.y);
-        double mc = 0.9 * (zero ? 0.0 : dxm);
-        if (fabs(mc) > 0.08) mc = 0.08 * (mc > 0 ? 1.0 : -1.0);
-        double const vm = ref_clamp_magnitude(old + mc);
+        double mc = sconst(0.9) * (zero ? 0.0 : dxm);
+        double const mmax = sconst(0.08);
+        if (fabs(mc) > mmax) mc = mmax * (mc > 0 ? 1.0 : -1.0);
+        double const vm = ref_clamp_magnitude(old + mc, sconst(1.5));
         // a NaN magnitude scales V to NaN, as the reference's polar rebuild does
         v = !(old <= 0.0) ? make_double2(v.x * (vm / old), v.y * (vm / old)) : make_double2(vm, 0.0);
     }
@@ -336,7 +356,7 @@ struct TaskPre {
 template <typename T, bool BWD>
 __device__ inline TaskPre<T> load_pre(FusedArgs const& a, GridTask const& tk) {
     int const lane = lane_id(), cls = tk.cls_cnt >> 8;
-    int const gi = lane >> (2 + cls);  // 4 << cls lanes per front
+    int const gi = lane >> grid_lane_shift(cls);
     TaskPre<T> r;
     r.gf = ld_stream(a.gfront + tk.front_beg + gi);
     if constexpr (BWD) {
@@ -678,8 +698,8 @@ __device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArg
             else grid_factor_task<T, 16, 16>(a, fa, ma, tk, pre, g, front, vec, pool, 17);
             break;
         case 3:
-            if constexpr (BWD) grid_backward_task<T, 32, 32>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 33);
-            else grid_factor_task<T, 32, 32>(a, fa, ma, tk, pre, g, front, vec, pool, 33);
+            if constexpr (BWD) grid_backward_task<T, 32, 64>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 33);
+            else grid_factor_task<T, 32, 64>(a, fa, ma, tk, pre, g, front, vec, pool, 33);
             break;
         default:
             if constexpr (BWD) grid_backward_task<T, 64, 64>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, a.ld4);
@@ -777,10 +797,10 @@ __device__ inline double block_max(double m, double* red) {
     return m;
 }
 
-// One workgroup per network; dynamic LDS = NW * pool elements of T.  With
-// FP32 factors the kernel is held to 128 VGPRs (4 wavefronts per SIMD): the
-// memory waits dominate, so the extra resident wavefronts pay.  The FP64
-// kernel keeps its natural register count (2 wavefronts per SIMD).
+// One workgroup per network; dynamic LDS = NW * pool elements of T.  The
+// memory waits dominate, so resident wavefronts pay: with FP32 factors the
+// kernel is held to 96 VGPRs (5 wavefronts per SIMD; a few values spill), with
+// FP64 factors to 168 (3 per SIMD, its natural count).
 //
 // The task descriptors and level ranges (a.tasks, a.lvl_ptr) come again as
 // restrict arguments: the kernel writes no memory they could alias, so the
@@ -788,7 +808,7 @@ __device__ inline double block_max(double m, double* red) {
 // counter.  Read as vector loads of a uniform address, each was followed by a
 // wait for every vector load in flight, the next task's prefetches included.
 template <typename T, int NW>
-__global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 4 : 1)
+__global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 5 : 3)
     k_nr_grid(FusedArgs a, MisArgs ma, UpdArgs ua, FrontArgs fa, T* front, T* vec, const GridTask* __restrict__ tasks,
               const int* __restrict__ lvl_ptr) {
     constexpr int NT = NW * kWave;

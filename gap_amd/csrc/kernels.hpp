@@ -24,9 +24,9 @@ __device__ inline double reference_max(double max_nonnan, double f0) { return f0
 // (cpu_newton_raphson.cpp:493-496): each returns its first argument unless
 // the comparison holds, so a NaN magnitude stays NaN (fmax / fmin would
 // replace it by the bound).
-__device__ inline double ref_clamp_magnitude(double vm) {
+__device__ inline double ref_clamp_magnitude(double vm, double hi = 1.5) {
     vm = (vm < 0.5) ? 0.5 : vm;
-    return (1.5 < vm) ? 1.5 : vm;
+    return (hi < vm) ? hi : vm;
 }
 
 // ------------------------------------------------------------------ Y-bus
