@@ -245,7 +245,8 @@ struct DevFronts {
     DBuf<int> fz_order, fz_lvl_beg, fz_lvl_ptr;
     DBuf<GridTask> fz_tasks;
     DBuf<int4> fz_gfront;  // per front: storage offset, vector offset, m | p << 16, list position of its first pivot
-    DBuf<int4> fz_pairs;
+    DBuf<int2> fz_pairs;
+    DBuf<int> fz_pair_y;
     DBuf<double2> fz_ypair;  // Y value of each pair, refreshed at every solve
     long long fz_npairs = 0;
     // -F of each Newton equation (factor type), written by every mismatch
@@ -285,7 +286,13 @@ struct GridItems {
     std::vector<int> const& it_j;
     std::vector<int> const& it_y;
     std::vector<int> const& it_pos;
+    std::vector<int> const& bus_off;  // per instance, n_inst + 1 entries
+    long long y_nnz;                  // Y-bus entries of the plan
 };
+
+// The grid schedule's pair list holds bus indices local to their network in
+// 16 bits each, so it takes networks of up to this many buses.
+constexpr int kGridMaxBuses = 0xffff;
 
 // Grid schedule of a packed plan: tasks (runs of at most kGridPerTask fronts of
 // one instance, level and class) and the flat assembly lists each task indexes.
@@ -304,6 +311,8 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     d.fz_varpos.clear();
     d.fz_ok = nf > 0 && d.max_front <= kClassMax[kGridClasses - 1] && d.front_entries < kIntMax &&
               d.vec_entries < kIntMax && n_index < kIntMax && n_values < kIntMax;
+    for (int g = 0; g < G && d.fz_ok; ++g)
+        if (it.bus_off[g + 1] - it.bus_off[g] > kGridMaxBuses) d.fz_ok = false;
     if (!d.fz_ok) return;
     // Jacobian entry -> (front, position) from the arrowhead lists
     std::vector<int> jfront(n_values, -1), jdst(n_values, 0);
@@ -396,7 +405,8 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     if (static_cast<int>(tfront.size()) != nf) throw Error(GAPHIP_E_ARG, "grid tasks do not cover every front once");
     std::vector<int> lvl_ptr(1, 0);
     std::vector<GridTask> tasks;
-    std::vector<int4> pairs;
+    std::vector<int2> pairs;   // (local i | local j << 16, LDS position | entry mask << 16)
+    std::vector<int> pair_y;   // Y-bus slot of each pair
     std::vector<int2> rhs, ext, vx, sol;
     std::vector<int> fpos(nf, -1);               // per front: list position of its first pivot
     std::vector<int> varpos(n_index, -1);        // per Newton variable: list position of its pivot
@@ -427,7 +437,13 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
                     for (int e = 0; e < 4; ++e)
                         if ((mask >> e & 1) && !in_tile(r0 + (e >> 1), c0 + (e & 1)))
                             throw Error(GAPHIP_E_ARG, "Jacobian pair outside its front");
-                    pairs.push_back(make_int4(it.it_i[t], it.it_j[t], it.it_y[t], (tile + r0 * ld + c0) | (mask << 16)));
+                    int const b0 = it.bus_off[fm.inst];
+                    int const li = it.it_i[t] - b0, lj = it.it_j[t] - b0;
+                    if (li < 0 || lj < 0 || li > kGridMaxBuses || lj > kGridMaxBuses || li >= it.bus_off[fm.inst + 1] - b0 ||
+                        lj >= it.bus_off[fm.inst + 1] - b0)
+                        throw Error(GAPHIP_E_ARG, "pair bus outside its network");
+                    pairs.push_back(make_int2(li | (lj << 16), (tile + r0 * ld + c0) | (mask << 16)));
+                    pair_y.push_back(it.it_y[t]);
                 }
                 fpos[n] = static_cast<int>(rhs.size());
                 for (int t = 0; t < fm.p; ++t) {
@@ -498,8 +514,8 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
         }
     }
     // bounds of every gathered index and LDS position
-    for (auto const& pr : pairs)
-        if (pr.x < 0 || pr.y < 0 || pr.z < 0 || (pr.w & 0xffff) + d.fz_ld4 + 1 >= (1 << 16))
+    for (size_t q = 0; q < pairs.size(); ++q)
+        if (pair_y[q] < 0 || pair_y[q] >= it.y_nnz || (pairs[q].y & 0xffff) + d.fz_ld4 + 1 >= (1 << 16))
             throw Error(GAPHIP_E_ARG, "pair out of range");
     auto check2 = [&](std::vector<int2> const& v, long long src_max, char const* what) {
         for (auto const& e : v)
@@ -537,9 +553,10 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     // padding read by the grid kernel's unconditional loads (never executed)
     tasks.insert(tasks.end(), kTaskPad, GridTask{});
     gfront.insert(gfront.end(), kWave, make_int4(0, 0, 0, 0));
-    pairs.insert(pairs.end(), kListPad, make_int4(0, 0, 0, 0));
+    pairs.insert(pairs.end(), kListPad, make_int2(0, 0));
+    pair_y.insert(pair_y.end(), kListPad, 0);
     for (auto* v : {&rhs, &ext, &vx, &sol}) v->insert(v->end(), kListPad, make_int2(0, -1));
-    d.fz_list_bytes = static_cast<long long>(tasks.size()) * sizeof(GridTask) + pairs.size() * sizeof(int4) +
+    d.fz_list_bytes = static_cast<long long>(tasks.size()) * sizeof(GridTask) + pairs.size() * sizeof(int2) +
                       (rhs.size() + ext.size() + vx.size() + sol.size()) * sizeof(int2);
     d.fz_order.upload(order, s);
     d.fz_lvl_beg.upload(lvl_beg, s);
@@ -547,6 +564,7 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     d.fz_tasks.upload(tasks, s);
     d.fz_gfront.upload(gfront, s);
     d.fz_pairs.upload(pairs, s);
+    d.fz_pair_y.upload(pair_y, s);
     d.fz_npairs = static_cast<long long>(pairs.size());
     d.fz_ypair.alloc(d.fz_npairs);
     d.fz_fe.alloc(rhs.size());  // one per Newton equation; doubles: room for either factor type
@@ -1321,7 +1339,7 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
         int bus = P.node_bus[g][key >> 1];
         return P.var_off[g] + ((key & 1) ? P.np[g].mag[bus] : P.np[g].ang[bus]);
     };
-    GridItems items{it_i, it_j, it_y, it_pos};
+    GridItems items{it_i, it_j, it_y, it_pos, P.bus_off, P.y_nnz};
     pack_fronts(plans, ent_base, key_row, key_col, P.n_var, P.j_nnz, P.fr, s, &items);
     P.bus_off_d.upload(P.bus_off, s);
     P.var_off_d.upload(P.var_off, s);
@@ -1647,7 +1665,7 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
     if (max_it > 0) {
         DevFronts const& d = P.fr;
         hipLaunchKernelGGL(k_gather_ypair<T>, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
-                           static_cast<int>(d.fz_npairs), d.fz_pairs.p, P.y_pu.p,
+                           static_cast<int>(d.fz_npairs), d.fz_pair_y.p, P.y_pu.p,
                            reinterpret_cast<cplx_t<T>*>(d.fz_ypair.p));
         if (sizeof(T) == 4 && !P.vf.p) {
             P.vf.alloc(N);

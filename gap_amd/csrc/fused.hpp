@@ -58,7 +58,7 @@ struct FusedArgs {
     const int* lvl_ptr;   // per (network, level): tasks [lvl_ptr[e], lvl_ptr[e+1])
     const GridTask* tasks;
     const int4* gfront;   // per front: (storage offset, vector offset, m | p << 16, list position of its first pivot)
-    const int4* pairs;    // (i, j, Y slot, LDS position | entry mask << 16)
+    const int2* pairs;    // (i | j << 16 as buses of the network, LDS position | entry mask << 16)
     const void* ypair;    // per-unit Y value of each pair (the factor type's complex), in pair order (gathered once per solve)
     const void* fe;       // per Newton equation: -F (factor type), written by the mismatch pass
     const int2* rhs;      // (equation index, LDS position)
@@ -224,10 +224,13 @@ __device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B]
         if (b > 0 && !use_rest) break;
         double const sr = vi.x * cr[b] + vi.y * cim[b];
         double const si = vi.y * cr[b] - vi.x * cim[b];
-        a.s[i[b]] = make_double2(sr, si);
+        // S feeds the Jacobian's diagonal blocks only: in FP64, or as its FP32
+        // copy (nothing reads the FP64 S of an FP32-factor solve afterwards)
         if constexpr (sizeof(T) == 4) {
             a.vf[i[b]] = make_float2(static_cast<float>(vi.x), static_cast<float>(vi.y));
             a.sf[i[b]] = make_float2(static_cast<float>(sr), static_cast<float>(si));
+        } else {
+            a.s[i[b]] = make_double2(sr, si);
         }
         if (pe >= 0) {
             double const fp = sr - sp.x;
@@ -344,7 +347,7 @@ __device__ inline void grid_update_bus(UpdArgs const& a, int i, bool zero) {
 template <typename T>
 struct TaskPre {
     int4 gf;                 // front record of the lane's front
-    int4 pr[kPairChunks];    // pair list entries
+    int2 pr[kPairChunks];    // pair list entries
     cplx_t<T> yv[kPairChunks]; // their per-unit Y
     int2 ex[kExtChunks];     // extend-add entries
     int2 rq;                 // right-hand-side entry (equation, LDS position)
@@ -425,24 +428,25 @@ __device__ inline T pivot_div(T x, T d) {
 // eliminated right-hand side w of every front.
 template <typename T, int MM, int L>
 __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, GridTask const& tk,
-                                 TaskPre<T> const& pre, int g, T* front, T* vec, T* pool, int ld) {
+                                 TaskPre<T> const& pre, int g, int b0, T* front, T* vec, T* pool, int ld) {
     constexpr int G = kWave / L;
     int const lane = lane_id(), gi = lane / L, t = lane % L;
     int const rows = ld - 1, wbase = G * rows * ld, region = wbase + G * MM;
-    int4 const* pr = pre.pr;
+    int2 const* pr = pre.pr;
     int2 const* ex = pre.ex;
     int2 const vv = pre.vv;
     using C = cplx_t<T>;
     C const* yv = pre.yv;
-    // the Jacobian inputs: V and S in the factor type
+    // the Jacobian inputs: V and S in the factor type, from the network's
+    // first bus (the pair list holds buses of the network)
     C const* vsrc;
     C const* ssrc;
     if constexpr (sizeof(T) == 4) {
-        vsrc = ma.vf;
-        ssrc = ma.sf;
+        vsrc = ma.vf + b0;
+        ssrc = ma.sf + b0;
     } else {
-        vsrc = ma.v;
-        ssrc = ma.s;
+        vsrc = ma.v + b0;
+        ssrc = ma.s + b0;
     }
     C const* ypair = static_cast<C const*>(a.ypair);
     // gathers, unconditional: every list entry (the next task's and padding
@@ -460,11 +464,12 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     C vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) {
-        int const i0 = __builtin_amdgcn_readfirstlane(pr[k].x);
-        bool const diag = pr[k].x == pr[k].y;
-        vi[k] = vsrc[pok[k] ? pr[k].x : i0];
-        vj[k] = vsrc[pok[k] && !diag ? pr[k].y : i0];
-        si[k] = ssrc[pok[k] && diag ? pr[k].x : i0];
+        int const li = pr[k].x & 0xffff, lj = static_cast<int>(static_cast<unsigned>(pr[k].x) >> 16);
+        int const i0 = __builtin_amdgcn_readfirstlane(li);
+        bool const diag = li == lj;
+        vi[k] = vsrc[pok[k] ? li : i0];
+        vj[k] = vsrc[pok[k] && !diag ? lj : i0];
+        si[k] = ssrc[pok[k] && diag ? li : i0];
     }
     T exv[kExtChunks];
 #pragma unroll
@@ -484,11 +489,12 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
         for (int e = lane; e < (region + V - 1) / V; e += kWave) z[e] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     wave_sync();
-    auto put_pair = [&](bool ok, int4 const& p4, C y2, C pi2, C pj2, C s2) {
-        int const base = p4.w & 0xffff, mask = p4.w >> 16;
+    auto put_pair = [&](bool ok, int2 const& p2, C y2, C pi2, C pj2, C s2) {
+        int const base = p2.y & 0xffff, mask = p2.y >> 16;
         if (!ok || !mask) return;
         T pt, pv, qt, qv;
-        jac_block_rect<T>(p4.x == p4.y, y2, pi2, pj2, s2, pt, pv, qt, qv);
+        jac_block_rect<T>((p2.x & 0xffff) == static_cast<int>(static_cast<unsigned>(p2.x) >> 16), y2, pi2, pj2, s2, pt,
+                          pv, qt, qv);
         if (mask & 1) pool[base] = pt;
         if (mask & 2) pool[base + 1] = pv;
         if (mask & 4) pool[base + ld] = qt;
@@ -497,8 +503,9 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) put_pair(pok[k], pr[k], yv[k], vi[k], vj[k], si[k]);
     for (int q = tk.pair_beg + kPairChunks * kWave + lane; q < tk.pair_end; q += kWave) {
-        int4 const p4 = ld_stream(a.pairs + q);
-        put_pair(true, p4, ld_stream(ypair + q), vsrc[p4.x], vsrc[p4.y], ssrc[p4.x]);
+        int2 const p2 = ld_stream(a.pairs + q);
+        int const li = p2.x & 0xffff, lj = static_cast<int>(static_cast<unsigned>(p2.x) >> 16);
+        put_pair(true, p2, ld_stream(ypair + q), vsrc[li], vsrc[lj], ssrc[li]);
     }
     if (rok) pool[pre.rq.y] = rv;
     for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave) {
@@ -682,37 +689,37 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
 }
 
 template <typename T, bool BWD>
-__device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int g, GridTask const& tk,
+__device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int g, int b0, GridTask const& tk,
                                  TaskPre<T> const& pre, T* front, T* vec, T* pool) {
     switch (tk.cls_cnt >> 8) {
         case 0:
             if constexpr (BWD) grid_backward_task<T, 4, 4>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 5);
-            else grid_factor_task<T, 4, 4>(a, fa, ma, tk, pre, g, front, vec, pool, 5);
+            else grid_factor_task<T, 4, 4>(a, fa, ma, tk, pre, g, b0, front, vec, pool, 5);
             break;
         case 1:
             if constexpr (BWD) grid_backward_task<T, 8, 8>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 9);
-            else grid_factor_task<T, 8, 8>(a, fa, ma, tk, pre, g, front, vec, pool, 9);
+            else grid_factor_task<T, 8, 8>(a, fa, ma, tk, pre, g, b0, front, vec, pool, 9);
             break;
         case 2:
             if constexpr (BWD) grid_backward_task<T, 16, 16>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 17);
-            else grid_factor_task<T, 16, 16>(a, fa, ma, tk, pre, g, front, vec, pool, 17);
+            else grid_factor_task<T, 16, 16>(a, fa, ma, tk, pre, g, b0, front, vec, pool, 17);
             break;
         case 3:
             if constexpr (BWD) grid_backward_task<T, 32, 64>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 33);
-            else grid_factor_task<T, 32, 64>(a, fa, ma, tk, pre, g, front, vec, pool, 33);
+            else grid_factor_task<T, 32, 64>(a, fa, ma, tk, pre, g, b0, front, vec, pool, 33);
             break;
         default:
             if constexpr (BWD) grid_backward_task<T, 64, 64>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, a.ld4);
-            else grid_factor_task<T, 64, 64>(a, fa, ma, tk, pre, g, front, vec, pool, a.ld4);
+            else grid_factor_task<T, 64, 64>(a, fa, ma, tk, pre, g, b0, front, vec, pool, a.ld4);
     }
 }
 
 // Per-unit Y of every pair in pair order, in the factor type, once per solve.
 template <typename T>
-__global__ void __launch_bounds__(256) k_gather_ypair(int n, const int4* pairs, const double2* y, cplx_t<T>* ypair) {
+__global__ void __launch_bounds__(256) k_gather_ypair(int n, const int* pair_y, const double2* y, cplx_t<T>* ypair) {
     int const k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) {
-        double2 const v = y[pairs[k].z];
+        double2 const v = y[pair_y[k]];
         ypair[k].x = static_cast<T>(v.x);
         ypair[k].y = static_cast<T>(v.y);
     }
@@ -755,7 +762,7 @@ __device__ inline void wait_for_deps(int const* prog, int wave, GridTask const& 
 // are loaded.
 template <typename T, bool BWD, int NW>
 __device__ inline void run_sweep(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma,
-                                 const GridTask* __restrict__ tasks, int g, int wave, int lo, int hi, T* front,
+                                 const GridTask* __restrict__ tasks, int g, int b0, int wave, int lo, int hi, T* front,
                                  T* vec, T* pool, int* prog) {
     constexpr int S = BWD ? -NW : NW;
     int q = BWD ? hi - 1 - wave : lo + wave;
@@ -768,14 +775,14 @@ __device__ inline void run_sweep(FusedArgs const& a, FrontArgs const& fa, MisArg
         wait_for_deps<BWD, NW>(prog, wave, t0);
         TaskPre<T> const p1 = load_pre<T, BWD>(a, t1);
         GridTask const t2 = desc(q + 2 * S);
-        grid_task<T, BWD>(a, fa, ma, g, t0, p0, front, vec, pool);
+        grid_task<T, BWD>(a, fa, ma, g, b0, t0, p0, front, vec, pool);
         q += S;
         publish_progress(prog, wave, q);
         if (!inside(q)) break;
         wait_for_deps<BWD, NW>(prog, wave, t1);
         TaskPre<T> const p2 = load_pre<T, BWD>(a, t2);
         GridTask const t3 = desc(q + 2 * S);
-        grid_task<T, BWD>(a, fa, ma, g, t1, p1, front, vec, pool);
+        grid_task<T, BWD>(a, fa, ma, g, b0, t1, p1, front, vec, pool);
         q += S;
         publish_progress(prog, wave, q);
         if (!inside(q)) break;
@@ -851,9 +858,9 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 5 : 3)
             break;
         }
         // (block_max's barriers also published the progress counters)
-        run_sweep<T, false, NW>(a, fa, ma, tasks, g, wave, t_lo, t_hi, front, vec, pool, prog_f);
+        run_sweep<T, false, NW>(a, fa, ma, tasks, g, b0, wave, t_lo, t_hi, front, vec, pool, prog_f);
         __syncthreads();
-        run_sweep<T, true, NW>(a, fa, ma, tasks, g, wave, t_lo, t_hi, front, vec, pool, prog_b);
+        run_sweep<T, true, NW>(a, fa, ma, tasks, g, b0, wave, t_lo, t_hi, front, vec, pool, prog_b);
         __syncthreads();
         bool const zero = a.lu_fail[g] != 0;
         for (int i = b0 + tid; i < b1; i += NT) grid_update_bus<T>(ua, i, zero);
