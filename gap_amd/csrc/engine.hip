@@ -258,7 +258,8 @@ struct DevFronts {
     // list position of each Newton variable, read once by the plan
     DBuf<double> fz_xs;
     std::vector<int> fz_varpos;
-    DBuf<int2> fz_rhs, fz_ext, fz_vx, fz_sol;
+    DBuf<unsigned> fz_rhs, fz_ext, fz_vx, fz_sol;  // index | LDS position << kListIdxBits
+    DBuf<int4> fz_netbase;  // per network: first front storage entry, first vector entry, first list entry
     long long fz_list_bytes = 0;
     std::vector<long long> fz_work;  // front slots + factor entries of fz_order's networks (descending)
 
@@ -287,6 +288,7 @@ struct GridItems {
     std::vector<int> const& it_y;
     std::vector<int> const& it_pos;
     std::vector<int> const& bus_off;  // per instance, n_inst + 1 entries
+    std::vector<int> const& var_off;  // per instance first Newton equation / variable, n_inst + 1 entries
     long long y_nnz;                  // Y-bus entries of the plan
 };
 
@@ -410,8 +412,41 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     std::vector<int2> rhs, ext, vx, sol;
     std::vector<int> fpos(nf, -1);               // per front: list position of its first pivot
     std::vector<int> varpos(n_index, -1);        // per Newton variable: list position of its pivot
+    // The grid schedule's own storage layout: fronts and slot vectors in task
+    // order, so each network's are contiguous and every list entry indexes
+    // from its network's base (netbase: fronts, vectors, right-hand-side list).
+    std::vector<long long> goff(nf, -1), gvoff(nf, -1);
+    std::vector<int4> netbase(G, make_int4(0, 0, 0, 0));
+    {
+        long long fo = 0, vo = 0;
+        int k = 0;
+        for (int g = 0; g < G; ++g) {
+            netbase[g].x = static_cast<int>(fo);
+            netbase[g].y = static_cast<int>(vo);
+            for (int e = lvl_beg[g]; e < lvl_beg[g + 1]; ++e)
+                for (int2 const& tr : per[e])
+                    for (int gi = 0; gi < (tr.y & 0xff); ++gi, ++k) {
+                        int const n = tfront[tr.x + gi];
+                        if (n < 0 || n >= nf || meta[n].inst != g || tr.x + gi != k)
+                            throw Error(GAPHIP_E_ARG, "grid tasks out of network order");
+                        goff[n] = fo;
+                        gvoff[n] = vo;
+                        fo += static_cast<long long>(meta[n].m) * meta[n].m;
+                        vo += meta[n].m;
+                    }
+            if (fo - netbase[g].x >= (1LL << kListIdxBits) || vo - netbase[g].y >= (1LL << kListIdxBits) ||
+                it.var_off[g + 1] - it.var_off[g] >= (1 << kListIdxBits)) {
+                d.fz_ok = false;  // a network too large for the 4-byte list entries
+                return;
+            }
+        }
+        if (k != nf || fo > d.front_entries || vo > d.vec_entries) throw Error(GAPHIP_E_ARG, "grid storage layout");
+    }
     int pool = 1;
-    for (auto const& lv : per) {
+    for (int g = 0; g < G; ++g) {
+      netbase[g].z = static_cast<int>(rhs.size());
+      for (int e = lvl_beg[g]; e < lvl_beg[g + 1]; ++e) {
+        auto const& lv = per[e];
         for (int2 const& tr : lv) {
             int const c = tr.y >> 8, cnt = tr.y & 0xff, L = kClassMax[c], Gc = kGridPerTask[c];
             int const ld = c == kGridClasses - 1 ? d.fz_ld4 : L + 1, rows = ld - 1;
@@ -450,7 +485,7 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
                     int const v = slot_col[fm.slot_beg + t];
                     if (v < 0 || v >= n_index || varpos[v] >= 0) throw Error(GAPHIP_E_ARG, "pivot variable out of range");
                     varpos[v] = static_cast<int>(rhs.size());
-                    rhs.push_back(make_int2(slot_row[fm.slot_beg + t], wpos + t));
+                    rhs.push_back(make_int2(slot_row[fm.slot_beg + t] - it.var_off[g], wpos + t));
                 }
                 // (variable index here; its pivot's list position once every task is built)
                 for (int t = fm.p; t < fm.m; ++t) sol.push_back(make_int2(slot_col[fm.slot_beg + t], wpos + t));
@@ -458,13 +493,16 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
                     ChildInfo const& ch = cinfo[ci];
                     int const nu = ch.mc - ch.pc;
                     int const* rl = rel.data() + ch.rel_off;
+                    if (ch.id < 0 || ch.id >= nf || meta[ch.id].inst != g || meta[ch.id].m != ch.mc)
+                        throw Error(GAPHIP_E_ARG, "child front out of range");
+                    long long const cf = goff[ch.id] - netbase[g].x, cv = gvoff[ch.id] - netbase[g].y;
                     for (int r = 0; r < nu; ++r) {
                         if (rl[r] < 0 || rl[r] >= fm.m) throw Error(GAPHIP_E_ARG, "extend-add map out of range");
                         for (int c2 = 0; c2 < nu; ++c2)
-                            ext.push_back(make_int2(static_cast<int>(ch.off + static_cast<long long>(ch.pc + r) * ch.mc +
+                            ext.push_back(make_int2(static_cast<int>(cf + static_cast<long long>(ch.pc + r) * ch.mc +
                                                                      ch.pc + c2),
                                                     tile + rl[r] * ld + rl[c2]));
-                        vx.push_back(make_int2(static_cast<int>(ch.voff + ch.pc + r), wpos + rl[r]));
+                        vx.push_back(make_int2(static_cast<int>(cv + ch.pc + r), wpos + rl[r]));
                     }
                 }
             }
@@ -477,12 +515,18 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
             tasks.push_back(tk);
         }
         lvl_ptr.push_back(static_cast<int>(tasks.size()));
+      }
     }
     // the backward tasks gather the solution of each structure slot at its
     // pivot's list position
-    for (auto& e : sol) {
-        if (e.x < 0 || e.x >= n_index || varpos[e.x] < 0) throw Error(GAPHIP_E_ARG, "structure variable without a pivot");
-        e.x = varpos[e.x];
+    for (int g = 0; g < G; ++g) {
+        int const t0 = lvl_ptr[lvl_beg[g]], t1 = lvl_ptr[lvl_beg[g + 1]];
+        if (t0 == t1) continue;
+        for (int q = tasks[t0].sol_beg; q < tasks[t1 - 1].sol_end; ++q) {
+            int2& e = sol[q];
+            if (e.x < 0 || e.x >= n_index || varpos[e.x] < netbase[g].z) throw Error(GAPHIP_E_ARG, "structure variable without a pivot");
+            e.x = varpos[e.x] - netbase[g].z;
+        }
     }
     // Dependencies between the tasks of a network: the task holding the last
     // child (factorisation) and the first parent (backward substitution) of
@@ -517,17 +561,26 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     for (size_t q = 0; q < pairs.size(); ++q)
         if (pair_y[q] < 0 || pair_y[q] >= it.y_nnz || (pairs[q].y & 0xffff) + d.fz_ld4 + 1 >= (1 << 16))
             throw Error(GAPHIP_E_ARG, "pair out of range");
-    auto check2 = [&](std::vector<int2> const& v, long long src_max, char const* what) {
+    // list entries index from their network's base (below 2^kListIdxBits by
+    // the layout check above; the bases leave room for every network after
+    // them) and hold LDS positions inside the pool
+    if (pool >= (1 << kListPosBits)) throw Error(GAPHIP_E_ARG, "grid task region too large");
+    auto check2 = [&](std::vector<int2> const& v, char const* what) {
         for (auto const& e : v)
-            if (e.x < 0 || e.x >= src_max || e.y < 0 || e.y >= pool) throw Error(GAPHIP_E_ARG, std::string(what) + " out of range");
+            if (e.x < 0 || e.x >= (1 << kListIdxBits) || e.y < 0 || e.y >= pool)
+                throw Error(GAPHIP_E_ARG, std::string(what) + " out of range");
     };
-    check2(rhs, n_index, "rhs entry");
-    check2(sol, static_cast<long long>(rhs.size()), "solution entry");
-    check2(ext, d.front_entries, "extend-add entry");
-    check2(vx, d.vec_entries, "vector entry");
+    check2(rhs, "rhs entry");
+    check2(sol, "solution entry");
+    check2(ext, "extend-add entry");
+    check2(vx, "vector entry");
     d.fz_pool = (pool + 3) / 4 * 4;  // whole 16-byte units per wavefront (the task clears its region with 16-byte stores)
     d.fz_rhs_eq.resize(rhs.size());
-    for (size_t q = 0; q < rhs.size(); ++q) d.fz_rhs_eq[q] = rhs[q].x;
+    for (int g = 0; g < G; ++g) {
+        int const t0 = lvl_ptr[lvl_beg[g]], t1 = lvl_ptr[lvl_beg[g + 1]];
+        if (t0 < t1)
+            for (int q = tasks[t0].rhs_beg; q < tasks[t1 - 1].rhs_end; ++q) d.fz_rhs_eq[q] = rhs[q].x + it.var_off[g];
+    }
     d.fz_varpos = std::move(varpos);
     // largest networks first: they bound the launch
     std::vector<int> order(G);
@@ -543,21 +596,32 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     for (int k = 0; k < nf; ++k) {
         int const n = tfront[k];
         FrontMeta const& fm = meta[n];
-        if (fm.off < 0 || fm.off > kIntMax || fm.voff < 0 || fm.voff > kIntMax || fm.m < 0 || fm.m > 0x7fff ||
-            fm.p < 0 || fm.p > fm.m)
+        int const g = fm.inst;
+        long long const lo = goff[n] - netbase[g].x, lv = gvoff[n] - netbase[g].y;
+        if (lo < 0 || lo + static_cast<long long>(fm.m) * fm.m > (1LL << kListIdxBits) || lv < 0 ||
+            lv + fm.m > (1LL << kListIdxBits) || fm.m < 0 || fm.m > 0x7fff || fm.p < 0 || fm.p > fm.m)
             throw Error(GAPHIP_E_ARG, "front record out of range");
-        if (fm.p > 0 && (fpos[n] < 0 || fpos[n] + fm.p > static_cast<long long>(rhs.size())))
+        if (fm.p > 0 && (fpos[n] < netbase[g].z || fpos[n] + fm.p > static_cast<long long>(rhs.size())))
             throw Error(GAPHIP_E_ARG, "front pivots outside the right-hand-side list");
-        gfront[k] = make_int4(static_cast<int>(fm.off), static_cast<int>(fm.voff), fm.m | (fm.p << 16), std::max(fpos[n], 0));
+        // storage, vector and list offsets from the network's bases
+        gfront[k] = make_int4(static_cast<int>(lo), static_cast<int>(lv), fm.m | (fm.p << 16),
+                              std::max(fpos[n] - netbase[g].z, 0));
     }
     // padding read by the grid kernel's unconditional loads (never executed)
     tasks.insert(tasks.end(), kTaskPad, GridTask{});
     gfront.insert(gfront.end(), kWave, make_int4(0, 0, 0, 0));
     pairs.insert(pairs.end(), kListPad, make_int2(0, 0));
     pair_y.insert(pair_y.end(), kListPad, 0);
-    for (auto* v : {&rhs, &ext, &vx, &sol}) v->insert(v->end(), kListPad, make_int2(0, -1));
+    // 4-byte list entries: index | LDS position << kListIdxBits; padding 0
+    auto pack = [](std::vector<int2> const& v) {
+        std::vector<unsigned> u(v.size() + kListPad, 0u);
+        for (size_t q = 0; q < v.size(); ++q)
+            u[q] = static_cast<unsigned>(v[q].x) | (static_cast<unsigned>(v[q].y) << kListIdxBits);
+        return u;
+    };
+    std::vector<unsigned> const rhs4 = pack(rhs), ext4 = pack(ext), vx4 = pack(vx), sol4 = pack(sol);
     d.fz_list_bytes = static_cast<long long>(tasks.size()) * sizeof(GridTask) + pairs.size() * sizeof(int2) +
-                      (rhs.size() + ext.size() + vx.size() + sol.size()) * sizeof(int2);
+                      (rhs4.size() + ext4.size() + vx4.size() + sol4.size()) * sizeof(unsigned);
     d.fz_order.upload(order, s);
     d.fz_lvl_beg.upload(lvl_beg, s);
     d.fz_lvl_ptr.upload(lvl_ptr, s);
@@ -571,10 +635,11 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     HIPCHK(hipMemsetAsync(d.fz_fe.p, 0, sizeof(double) * rhs.size(), s));
     d.fz_xs.alloc(rhs.size() + kVecPad);
     HIPCHK(hipMemsetAsync(d.fz_xs.p, 0, sizeof(double) * (rhs.size() + kVecPad), s));
-    d.fz_rhs.upload(rhs, s);
-    d.fz_ext.upload(ext, s);
-    d.fz_vx.upload(vx, s);
-    d.fz_sol.upload(sol, s);
+    d.fz_rhs.upload(rhs4, s);
+    d.fz_ext.upload(ext4, s);
+    d.fz_vx.upload(vx4, s);
+    d.fz_sol.upload(sol4, s);
+    d.fz_netbase.upload(netbase, s);
     HIPCHK(hipStreamSynchronize(s));
 }
 
@@ -1339,7 +1404,7 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
         int bus = P.node_bus[g][key >> 1];
         return P.var_off[g] + ((key & 1) ? P.np[g].mag[bus] : P.np[g].ang[bus]);
     };
-    GridItems items{it_i, it_j, it_y, it_pos, P.bus_off, P.y_nnz};
+    GridItems items{it_i, it_j, it_y, it_pos, P.bus_off, P.var_off, P.y_nnz};
     pack_fronts(plans, ent_base, key_row, key_col, P.n_var, P.j_nnz, P.fr, s, &items);
     P.bus_off_d.upload(P.bus_off, s);
     P.var_off_d.upload(P.var_off, s);
@@ -1672,7 +1737,7 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
             P.sf.alloc(N);
         }
         FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, P.var_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
-                     d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_fe.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_pool, d.fz_ld4,
+                     d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_fe.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_netbase.p, d.fz_pool, d.fz_ld4,
                      P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, d.fz_xs.p};
         MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
                    P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p, d.fz_fe.p, P.vf.p, P.sf.p};

@@ -57,14 +57,18 @@ struct FusedArgs {
     const int* lvl_beg;   // per network: its levels are [lvl_beg[g], lvl_beg[g+1]) of lvl_ptr
     const int* lvl_ptr;   // per (network, level): tasks [lvl_ptr[e], lvl_ptr[e+1])
     const GridTask* tasks;
-    const int4* gfront;   // per front: (storage offset, vector offset, m | p << 16, list position of its first pivot)
+    const int4* gfront;   // per front, from its network's bases: (storage offset, vector offset, m | p << 16, list position of its first pivot)
     const int2* pairs;    // (i | j << 16 as buses of the network, LDS position | entry mask << 16)
     const void* ypair;    // per-unit Y value of each pair (the factor type's complex), in pair order (gathered once per solve)
     const void* fe;       // per Newton equation: -F (factor type), written by the mismatch pass
-    const int2* rhs;      // (equation index, LDS position)
-    const int2* ext;      // (front storage index, LDS position)
-    const int2* vx;       // (vector index, LDS position)
-    const int2* sol;      // (variable index, LDS position)
+    // list entries index | LDS position << kListIdxBits, the index counted
+    // from the network's base (netbase for fronts, vectors and the list,
+    // var_off for equations)
+    const unsigned* rhs;  // (equation, LDS position)
+    const unsigned* ext;  // (front storage entry, LDS position)
+    const unsigned* vx;   // (vector entry, LDS position)
+    const unsigned* sol;  // (list position of the variable's pivot, LDS position)
+    const int4* netbase;  // per network: first front storage entry, first vector entry, first list entry
     int pool;             // LDS elements per wavefront
     int ld4;              // leading dimension of the 64-slot class tiles (largest such front + 1)
     unsigned long long* maxf;  // rows of per-network max |F| per iteration
@@ -129,6 +133,12 @@ __device__ inline int lane_id() {
     asm volatile("" : "+v"(l));
     return l & (kWave - 1);
 }
+
+// Grid list entries: an index below 2^kListIdxBits and an LDS position below
+// 2^kListPosBits in 32 bits.
+constexpr int kListIdxBits = 19, kListPosBits = 32 - kListIdxBits;
+__device__ inline int lidx(unsigned u) { return static_cast<int>(u & ((1u << kListIdxBits) - 1u)); }
+__device__ inline int lpos(unsigned u) { return static_cast<int>(u >> kListIdxBits); }
 
 // Lanes per front of each grid task class, as a shift: 4, 8 and 16 lanes
 // for fronts of up to 4, 8 and 16 slots (16, 8 and 4 fronts per task), a
@@ -349,10 +359,10 @@ struct TaskPre {
     int4 gf;                 // front record of the lane's front
     int2 pr[kPairChunks];    // pair list entries
     cplx_t<T> yv[kPairChunks]; // their per-unit Y
-    int2 ex[kExtChunks];     // extend-add entries
-    int2 rq;                 // right-hand-side entry (equation, LDS position)
-    int2 vv;                 // child-vector entry
-    int2 so;                 // (backward) solution entries
+    unsigned ex[kExtChunks]; // extend-add entries
+    unsigned rq;             // right-hand-side entry
+    unsigned vv;             // child-vector entry
+    unsigned so;             // (backward) solution entry
 };
 
 // Unconditional: entries past the task's lists are the next task's or padding.
@@ -428,13 +438,13 @@ __device__ inline T pivot_div(T x, T d) {
 // eliminated right-hand side w of every front.
 template <typename T, int MM, int L>
 __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, GridTask const& tk,
-                                 TaskPre<T> const& pre, int g, int b0, T* front, T* vec, T* pool, int ld) {
+                                 TaskPre<T> const& pre, int g, int b0, int r0, T* front, T* vec, T* pool, int ld) {
     constexpr int G = kWave / L;
     int const lane = lane_id(), gi = lane / L, t = lane % L;
     int const rows = ld - 1, wbase = G * rows * ld, region = wbase + G * MM;
     int2 const* pr = pre.pr;
-    int2 const* ex = pre.ex;
-    int2 const vv = pre.vv;
+    unsigned const* ex = pre.ex;
+    unsigned const vv = pre.vv;
     using C = cplx_t<T>;
     C const* yv = pre.yv;
     // the Jacobian inputs: V and S in the factor type, from the network's
@@ -456,9 +466,9 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) pok[k] = tk.pair_beg + k * kWave + lane < tk.pair_end;
 #pragma unroll
-    for (int k = 0; k < kExtChunks; ++k) eok[k] = tk.ext_beg + k * kWave + lane < tk.ext_end && ex[k].y >= 0;
-    bool const rok = tk.rhs_beg + lane < tk.rhs_end && pre.rq.y >= 0;
-    bool const vok = tk.vx_beg + lane < tk.vx_end && vv.y >= 0;
+    for (int k = 0; k < kExtChunks; ++k) eok[k] = tk.ext_beg + k * kWave + lane < tk.ext_end;
+    bool const rok = tk.rhs_beg + lane < tk.rhs_end;
+    bool const vok = tk.vx_beg + lane < tk.vx_end;
     // a lane whose value is not used reads lane 0's element instead, so its
     // load falls into a line the wavefront fetches anyway
     C vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
@@ -474,10 +484,10 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     T exv[kExtChunks];
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k)
-        exv[k] = ld_stream(front + (eok[k] ? ex[k].x : __builtin_amdgcn_readfirstlane(ex[k].x)));
-    T const vxv = vec[vok ? vv.x : __builtin_amdgcn_readfirstlane(vv.x)];
-    T const* fe = static_cast<T const*>(a.fe);
-    T const rv = fe[rok ? pre.rq.x : __builtin_amdgcn_readfirstlane(pre.rq.x)];
+        exv[k] = ld_stream(front + (eok[k] ? lidx(ex[k]) : __builtin_amdgcn_readfirstlane(lidx(ex[k]))));
+    T const vxv = vec[vok ? lidx(vv) : __builtin_amdgcn_readfirstlane(lidx(vv))];
+    T const* fe = static_cast<T const*>(a.fe) + r0;  // the network's equations
+    T const rv = fe[rok ? lidx(pre.rq) : __builtin_amdgcn_readfirstlane(lidx(pre.rq))];
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
     wave_sync();  // the previous task of this wavefront has finished with the region
@@ -507,23 +517,23 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
         int const li = p2.x & 0xffff, lj = static_cast<int>(static_cast<unsigned>(p2.x) >> 16);
         put_pair(true, p2, ld_stream(ypair + q), vsrc[li], vsrc[lj], ssrc[li]);
     }
-    if (rok) pool[pre.rq.y] = rv;
+    if (rok) pool[lpos(pre.rq)] = rv;
     for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave) {
-        int2 const e = ld_stream(a.rhs + q);
-        pool[e.y] = fe[e.x];
+        unsigned const e = ld_stream(a.rhs + q);
+        pool[lpos(e)] = fe[lidx(e)];
     }
     wave_sync();
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k)
-        if (eok[k]) lds_add(pool + ex[k].y, exv[k]);
+        if (eok[k]) lds_add(pool + lpos(ex[k]), exv[k]);
     for (int q = tk.ext_beg + kExtChunks * kWave + lane; q < tk.ext_end; q += kWave) {
-        int2 const e = ld_stream(a.ext + q);
-        lds_add(pool + e.y, ld_stream(front + e.x));
+        unsigned const e = ld_stream(a.ext + q);
+        lds_add(pool + lpos(e), ld_stream(front + lidx(e)));
     }
-    if (vok) lds_add(pool + vv.y, vxv);
+    if (vok) lds_add(pool + lpos(vv), vxv);
     for (int q = tk.vx_beg + kWave + lane; q < tk.vx_end; q += kWave) {
-        int2 const e = ld_stream(a.vx + q);
-        lds_add(pool + e.y, vec[e.x]);
+        unsigned const e = ld_stream(a.vx + q);
+        lds_add(pool + lpos(e), vec[lidx(e)]);
     }
     wave_sync();
     bool const act = gi < (tk.cls_cnt & 0xff);
@@ -605,14 +615,14 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
     constexpr int G = kWave / L, CH = MM < 16 ? MM : 16;
     int const lane = lane_id(), gi = lane / L, t = lane % L;
     int const rows = ld - 1, wbase = G * rows * ld;
-    int2 const so = pre.so;
-    bool const sok = tk.sol_beg + lane < tk.sol_end && so.y >= 0;
+    unsigned const so = pre.so;
+    bool const sok = tk.sol_beg + lane < tk.sol_end;
     bool const act = gi < (tk.cls_cnt & 0xff);
     int const m = act ? pre.gf.z & 0xffff : 0, p = act ? pre.gf.z >> 16 : 0;
     // unconditional loads: the front record of an inactive lane is another
     // front's or padding
     // (a lane whose value is not used reads lane 0's element: same line)
-    T const sv = xsol[sok ? so.x : __builtin_amdgcn_readfirstlane(so.x)];
+    T const sv = xsol[sok ? lidx(so) : __builtin_amdgcn_readfirstlane(lidx(so))];
     T const wv = vec[t < p ? pre.gf.y + t : __builtin_amdgcn_readfirstlane(pre.gf.y)];
     int const p_max = wave_group_max<L>(p);
     // small fronts: the U rows by columns (lane t reads column t of each pivot
@@ -630,10 +640,10 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
     wave_sync();  // the previous task of this wavefront has finished with the region
-    if (sok) pool[so.y] = sv;
+    if (sok) pool[lpos(so)] = sv;
     for (int q = tk.sol_beg + kWave + lane; q < tk.sol_end; q += kWave) {
-        int2 const e = ld_stream(a.sol + q);
-        pool[e.y] = xsol[e.x];
+        unsigned const e = ld_stream(a.sol + q);
+        pool[lpos(e)] = xsol[lidx(e)];
     }
     if (t < p) w[t] = wv;
     T xme = T(0);  // this lane's solution value (t < p)
@@ -689,28 +699,28 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
 }
 
 template <typename T, bool BWD>
-__device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int g, int b0, GridTask const& tk,
-                                 TaskPre<T> const& pre, T* front, T* vec, T* pool) {
+__device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int g, int b0, int r0, GridTask const& tk,
+                                 TaskPre<T> const& pre, T* front, T* vec, T* xs, T* pool) {
     switch (tk.cls_cnt >> 8) {
         case 0:
-            if constexpr (BWD) grid_backward_task<T, 4, 4>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 5);
-            else grid_factor_task<T, 4, 4>(a, fa, ma, tk, pre, g, b0, front, vec, pool, 5);
+            if constexpr (BWD) grid_backward_task<T, 4, 4>(a, fa, tk, pre, front, vec, xs, pool, 5);
+            else grid_factor_task<T, 4, 4>(a, fa, ma, tk, pre, g, b0, r0, front, vec, pool, 5);
             break;
         case 1:
-            if constexpr (BWD) grid_backward_task<T, 8, 8>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 9);
-            else grid_factor_task<T, 8, 8>(a, fa, ma, tk, pre, g, b0, front, vec, pool, 9);
+            if constexpr (BWD) grid_backward_task<T, 8, 8>(a, fa, tk, pre, front, vec, xs, pool, 9);
+            else grid_factor_task<T, 8, 8>(a, fa, ma, tk, pre, g, b0, r0, front, vec, pool, 9);
             break;
         case 2:
-            if constexpr (BWD) grid_backward_task<T, 16, 16>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 17);
-            else grid_factor_task<T, 16, 16>(a, fa, ma, tk, pre, g, b0, front, vec, pool, 17);
+            if constexpr (BWD) grid_backward_task<T, 16, 16>(a, fa, tk, pre, front, vec, xs, pool, 17);
+            else grid_factor_task<T, 16, 16>(a, fa, ma, tk, pre, g, b0, r0, front, vec, pool, 17);
             break;
         case 3:
-            if constexpr (BWD) grid_backward_task<T, 32, 64>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, 33);
-            else grid_factor_task<T, 32, 64>(a, fa, ma, tk, pre, g, b0, front, vec, pool, 33);
+            if constexpr (BWD) grid_backward_task<T, 32, 64>(a, fa, tk, pre, front, vec, xs, pool, 33);
+            else grid_factor_task<T, 32, 64>(a, fa, ma, tk, pre, g, b0, r0, front, vec, pool, 33);
             break;
         default:
-            if constexpr (BWD) grid_backward_task<T, 64, 64>(a, fa, tk, pre, front, vec, static_cast<T*>(a.xs), pool, a.ld4);
-            else grid_factor_task<T, 64, 64>(a, fa, ma, tk, pre, g, b0, front, vec, pool, a.ld4);
+            if constexpr (BWD) grid_backward_task<T, 64, 64>(a, fa, tk, pre, front, vec, xs, pool, a.ld4);
+            else grid_factor_task<T, 64, 64>(a, fa, ma, tk, pre, g, b0, r0, front, vec, pool, a.ld4);
     }
 }
 
@@ -762,8 +772,8 @@ __device__ inline void wait_for_deps(int const* prog, int wave, GridTask const& 
 // are loaded.
 template <typename T, bool BWD, int NW>
 __device__ inline void run_sweep(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma,
-                                 const GridTask* __restrict__ tasks, int g, int b0, int wave, int lo, int hi, T* front,
-                                 T* vec, T* pool, int* prog) {
+                                 const GridTask* __restrict__ tasks, int g, int b0, int r0, int wave, int lo, int hi,
+                                 T* front, T* vec, T* xs, T* pool, int* prog) {
     constexpr int S = BWD ? -NW : NW;
     int q = BWD ? hi - 1 - wave : lo + wave;
     auto inside = [&](int k) { return BWD ? k >= lo : k < hi; };
@@ -775,14 +785,14 @@ __device__ inline void run_sweep(FusedArgs const& a, FrontArgs const& fa, MisArg
         wait_for_deps<BWD, NW>(prog, wave, t0);
         TaskPre<T> const p1 = load_pre<T, BWD>(a, t1);
         GridTask const t2 = desc(q + 2 * S);
-        grid_task<T, BWD>(a, fa, ma, g, b0, t0, p0, front, vec, pool);
+        grid_task<T, BWD>(a, fa, ma, g, b0, r0, t0, p0, front, vec, xs, pool);
         q += S;
         publish_progress(prog, wave, q);
         if (!inside(q)) break;
         wait_for_deps<BWD, NW>(prog, wave, t1);
         TaskPre<T> const p2 = load_pre<T, BWD>(a, t2);
         GridTask const t3 = desc(q + 2 * S);
-        grid_task<T, BWD>(a, fa, ma, g, b0, t1, p1, front, vec, pool);
+        grid_task<T, BWD>(a, fa, ma, g, b0, r0, t1, p1, front, vec, xs, pool);
         q += S;
         publish_progress(prog, wave, q);
         if (!inside(q)) break;
@@ -826,6 +836,11 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 5 : 3)
     int const b0 = a.bus_off[g], b1 = a.bus_off[g + 1];
     int const t_lo = lvl_ptr[a.lvl_beg[g]], t_hi = lvl_ptr[a.lvl_beg[g + 1]];  // the network's tasks
     int const r0 = a.var_off[g], r1 = a.var_off[g + 1];                       // and its Newton equations
+    // the network's fronts, vectors and solution list (the grid lists index from them)
+    int4 const nb = a.netbase[g];
+    T* const front_n = front + nb.x;
+    T* const vec_n = vec + nb.y;
+    T* const xs_n = static_cast<T*>(a.xs) + nb.z;
     int const tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     T* pool = reinterpret_cast<T*>(grid_smem) + wave * a.pool;
     for (int it = 0; it < a.max_it; ++it) {
@@ -858,9 +873,9 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 5 : 3)
             break;
         }
         // (block_max's barriers also published the progress counters)
-        run_sweep<T, false, NW>(a, fa, ma, tasks, g, b0, wave, t_lo, t_hi, front, vec, pool, prog_f);
+        run_sweep<T, false, NW>(a, fa, ma, tasks, g, b0, r0, wave, t_lo, t_hi, front_n, vec_n, xs_n, pool, prog_f);
         __syncthreads();
-        run_sweep<T, true, NW>(a, fa, ma, tasks, g, b0, wave, t_lo, t_hi, front, vec, pool, prog_b);
+        run_sweep<T, true, NW>(a, fa, ma, tasks, g, b0, r0, wave, t_lo, t_hi, front_n, vec_n, xs_n, pool, prog_b);
         __syncthreads();
         bool const zero = a.lu_fail[g] != 0;
         for (int i = b0 + tid; i < b1; i += NT) grid_update_bus<T>(ua, i, zero);
