@@ -2157,7 +2157,9 @@ int gaphip_plan_set_schedule(gaphip_plan* P, int schedule, int* effective) {
             throw Error(GAPHIP_E_ARG, "unknown schedule");
         if (schedule == GAPHIP_SCHED_GRID && !P->fr.fz_ok)
             throw Error(GAPHIP_E_ARG, "the grid schedule needs every front to have at most 64 slots (largest: " +
-                                          std::to_string(P->fr.max_front) + ")");
+                                          std::to_string(P->fr.max_front) +
+                                          ") and every network at most 65,535 buses and fewer than 2^19 front "
+                                          "storage and slot-vector entries");
         P->schedule = schedule;
         if (effective) *effective = P->grid_schedule() ? GAPHIP_SCHED_GRID : GAPHIP_SCHED_LEVELS;
     })

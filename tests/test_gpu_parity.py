@@ -185,6 +185,31 @@ def test_100k_grid_mixed_vs_fp64(device, oracle):
         assert np.abs(f).max() < 1e-8
 
 
+def test_network_above_grid_limits_runs_on_levels(device, oracle):
+    """A network of more than 65,535 buses (small fronts: a tree with 40 extra
+    edges) is outside the grid schedule's list formats: a batch holding it runs
+    on the level schedule, asking for the grid schedule is an argument error,
+    and every network still reaches a fixed point of the oracle's mismatch."""
+    big = gridgen.small_mesh(70_000, seed=5, n_extra=40)
+    nets = [gridgen.small_mesh(300, seed=6), big]
+    plan = capi.Plan(nets, precision="mixed")
+    assert plan.schedule == "levels"
+    with pytest.raises(capi.GapHipError):
+        plan.set_schedule("grid")
+    r = plan.solve(1e-8, 100, 100e6)
+    assert r["converged"].all()
+    for k, net in enumerate(nets):
+        y = oracle.build_ybus(net)
+        ypu = (y[0], y[1], y[2] * (net.u_rated[0] ** 2 / 100e6))
+        f = oracle.mismatches(net, r["v"][k], ypu) / 100e6
+        assert np.abs(f).max() < 1e-8
+    small = capi.Plan(nets[:1], precision="mixed")
+    assert small.schedule == "grid"
+    rs = small.solve(1e-8, 100, 100e6)
+    assert int(rs["iterations"][0]) == int(r["iterations"][0])
+    assert np.abs(rs["v"][0] - r["v"][0]).max() <= V_TOL
+
+
 @pytest.mark.parametrize("schedule", ["auto", "levels"])
 def test_mixed_precision_close_to_fp64(device, schedule):
     nets = [gridgen.small_mesh(n, seed=s) for n, s in [(200, 0), (800, 4), (3000, 9)]]
