@@ -161,12 +161,13 @@ constexpr int kPairChunks = 1, kExtChunks = 2;
 // therefore extend past their last element (padding entries are neutral:
 // index 0, LDS position -1, mask 0), and the bounds are applied where the
 // values are used.
-constexpr int kListPad = 2 * kWave;             // entries after each task list
+constexpr int kListPad = 4 * kWave;             // entries after each task list
 constexpr int kTaskPad = 16;                    // descriptors after the last task (2 ahead at 8 wavefronts)
 constexpr int kRowPad = 4;                      // Y-bus entries after the last row
 constexpr long long kFrontPad = kWave * kWave;  // front storage: one 64 x 64 front
 constexpr long long kVecPad = kWave;            // slot vectors and slot columns
-static_assert(kListPad >= kPairChunks * kWave && kListPad >= kExtChunks * kWave, "list padding too short");
+static_assert(kListPad >= kPairChunks * kWave && kListPad >= kExtChunks * kWave && kListPad >= 4 * kWave,
+              "list padding too short");
 
 // ---------------------------------------------------------------------------
 // Rectangular forms of the per-bus and per-pair arithmetic.  They equal the
@@ -512,10 +513,22 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     };
 #pragma unroll
     for (int k = 0; k < kPairChunks; ++k) put_pair(pok[k], pr[k], yv[k], vi[k], vj[k], si[k]);
-    for (int q = tk.pair_beg + kPairChunks * kWave + lane; q < tk.pair_end; q += kWave) {
-        int2 const p2 = ld_stream(a.pairs + q);
-        int const li = p2.x & 0xffff, lj = static_cast<int>(static_cast<unsigned>(p2.x) >> 16);
-        put_pair(true, p2, ld_stream(ypair + q), vsrc[li], vsrc[lj], ssrc[li]);
+    // entries past the register chunks, two per lane and step: both list
+    // entries are read before either gather (the list padding covers the
+    // second read past the list's end; a lane past the task's end gathers
+    // the step's first element)
+    for (int q0 = tk.pair_beg + kPairChunks * kWave; q0 < tk.pair_end; q0 += 2 * kWave) {
+        int const q = q0 + lane;
+        int2 const p0 = ld_stream(a.pairs + q), p1 = ld_stream(a.pairs + q + kWave);
+        C const y0 = ld_stream(ypair + q), y1 = ld_stream(ypair + q + kWave);
+        bool const ok0 = q < tk.pair_end, ok1 = q + kWave < tk.pair_end;
+        int const f = __builtin_amdgcn_readfirstlane(p0.x & 0xffff);
+        int const i0 = ok0 ? p0.x & 0xffff : f, j0 = ok0 ? static_cast<int>(static_cast<unsigned>(p0.x) >> 16) : f;
+        int const i1 = ok1 ? p1.x & 0xffff : f, j1 = ok1 ? static_cast<int>(static_cast<unsigned>(p1.x) >> 16) : f;
+        C const vi0 = vsrc[i0], vj0 = vsrc[j0], s0 = ssrc[i0];
+        C const vi1 = vsrc[i1], vj1 = vsrc[j1], s1 = ssrc[i1];
+        put_pair(ok0, p0, y0, vi0, vj0, s0);
+        put_pair(ok1, p1, y1, vi1, vj1, s1);
     }
     if (rok) pool[lpos(pre.rq)] = rv;
     for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave) {
@@ -526,14 +539,29 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k)
         if (eok[k]) lds_add(pool + lpos(ex[k]), exv[k]);
-    for (int q = tk.ext_beg + kExtChunks * kWave + lane; q < tk.ext_end; q += kWave) {
-        unsigned const e = ld_stream(a.ext + q);
-        lds_add(pool + lpos(e), ld_stream(front + lidx(e)));
+    // (the same for the child entries, four per lane and step)
+    for (int q0 = tk.ext_beg + kExtChunks * kWave; q0 < tk.ext_end; q0 += 4 * kWave) {
+        int const q = q0 + lane;
+        unsigned e[4];
+        T x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = ld_stream(a.ext + q + u * kWave);
+        int const f = __builtin_amdgcn_readfirstlane(lidx(e[0]));
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = ld_stream(front + (q + u * kWave < tk.ext_end ? lidx(e[u]) : f));
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (q + u * kWave < tk.ext_end) lds_add(pool + lpos(e[u]), x[u]);
     }
     if (vok) lds_add(pool + lpos(vv), vxv);
-    for (int q = tk.vx_beg + kWave + lane; q < tk.vx_end; q += kWave) {
-        unsigned const e = ld_stream(a.vx + q);
-        lds_add(pool + lpos(e), vec[lidx(e)]);
+    for (int q0 = tk.vx_beg + kWave; q0 < tk.vx_end; q0 += 2 * kWave) {
+        int const q = q0 + lane;
+        unsigned const e0 = ld_stream(a.vx + q), e1 = ld_stream(a.vx + q + kWave);
+        bool const ok0 = q < tk.vx_end, ok1 = q + kWave < tk.vx_end;
+        int const f = __builtin_amdgcn_readfirstlane(lidx(e0));
+        T const x0 = vec[ok0 ? lidx(e0) : f], x1 = vec[ok1 ? lidx(e1) : f];
+        if (ok0) lds_add(pool + lpos(e0), x0);
+        if (ok1) lds_add(pool + lpos(e1), x1);
     }
     wave_sync();
     bool const act = gi < (tk.cls_cnt & 0xff);
