@@ -1105,7 +1105,7 @@ struct gaphip_plan {
     gaphip::DBuf<int> bus_type_d;
     gaphip::DBuf<double> adj_lossb, adj_gp, adj_gq, adj_gr, adj_gx, adj_gvm, adj_gva;
     gaphip::DBuf<double2> v, vpol, s, sspec, y_pu, y_given;
-    gaphip::DBuf<float2> vf, sf;  // grid schedule with FP32 factors: V and S in FP32 (allocated on first use)
+    gaphip::DBuf<float4> vsf;  // grid schedule with FP32 factors: V and S in FP32 per bus (allocated on first use)
     std::vector<int> y_given_mode;  // per instance: 0 assemble, 1 siemens given, 2 per-unit given
     gaphip::DBuf<unsigned long long> maxf;
     gaphip::DBuf<double> front64, vec64;
@@ -1732,15 +1732,14 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
         hipLaunchKernelGGL(k_gather_ypair<T>, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
                            static_cast<int>(d.fz_npairs), d.fz_pair_y.p, P.y_pu.p,
                            reinterpret_cast<cplx_t<T>*>(d.fz_ypair.p));
-        if (sizeof(T) == 4 && !P.vf.p) {
-            P.vf.alloc(N);
-            P.sf.alloc(N);
+        if (sizeof(T) == 4 && !P.vsf.p) {
+            P.vsf.alloc(N);
         }
         FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, P.var_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
                      d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_fe.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_netbase.p, d.fz_pool, d.fz_ld4,
                      P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, d.fz_xs.p};
         MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
-                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p, d.fz_fe.p, P.vf.p, P.sf.p};
+                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p, d.fz_fe.p, P.vsf.p};
         UpdArgs ua{static_cast<int>(N), P.bus_inst.p, P.done.p, P.lu_fail.p, P.ang.p, P.mag.p, P.dx.p, P.v.p,
                    P.bpos.p, d.fz_xs.p};
         FrontArgs fra = d.args(nullptr, P.lu_fail.p);

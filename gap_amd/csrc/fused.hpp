@@ -92,30 +92,6 @@ __device__ inline int wave_group_max(int v) {
     return v;
 }
 
-// Plan lists, and front data read once per iteration, stream past the caches
-// so that the per-network bus arrays stay resident.
-template <typename T>
-__device__ inline T ld_stream(const T* p) {
-    return __builtin_nontemporal_load(p);
-}
-__device__ inline int4 ld_stream(const int4* p) {
-    const int* q = reinterpret_cast<const int*>(p);
-    return make_int4(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1), __builtin_nontemporal_load(q + 2),
-                     __builtin_nontemporal_load(q + 3));
-}
-__device__ inline int2 ld_stream(const int2* p) {
-    const int* q = reinterpret_cast<const int*>(p);
-    return make_int2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
-}
-__device__ inline double2 ld_stream(const double2* p) {
-    const double* q = reinterpret_cast<const double*>(p);
-    return make_double2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
-}
-__device__ inline float2 ld_stream(const float2* p) {
-    const float* q = reinterpret_cast<const float*>(p);
-    return make_float2(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
-}
-
 // Complex of the factor type: the Jacobian blocks of the grid kernel are
 // evaluated in it (FP32 factors: from FP32 copies of V, S and Y, since every
 // entry is rounded to FP32 for the factorisation anyway; the mismatch, the
@@ -238,8 +214,8 @@ __device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B]
         // S feeds the Jacobian's diagonal blocks only: in FP64, or as its FP32
         // copy (nothing reads the FP64 S of an FP32-factor solve afterwards)
         if constexpr (sizeof(T) == 4) {
-            a.vf[i[b]] = make_float2(static_cast<float>(vi.x), static_cast<float>(vi.y));
-            a.sf[i[b]] = make_float2(static_cast<float>(sr), static_cast<float>(si));
+            a.vsf[i[b]] = make_float4(static_cast<float>(vi.x), static_cast<float>(vi.y), static_cast<float>(sr),
+                                      static_cast<float>(si));
         } else {
             a.s[i[b]] = make_double2(sr, si);
         }
@@ -372,19 +348,19 @@ __device__ inline TaskPre<T> load_pre(FusedArgs const& a, GridTask const& tk) {
     int const lane = lane_id(), cls = tk.cls_cnt >> 8;
     int const gi = lane >> grid_lane_shift(cls);
     TaskPre<T> r;
-    r.gf = ld_stream(a.gfront + tk.front_beg + gi);
+    r.gf = a.gfront[tk.front_beg + gi];
     if constexpr (BWD) {
-        r.so = ld_stream(a.sol + tk.sol_beg + lane);
+        r.so = a.sol[tk.sol_beg + lane];
     } else {
 #pragma unroll
         for (int k = 0; k < kPairChunks; ++k) {
-            r.pr[k] = ld_stream(a.pairs + tk.pair_beg + k * kWave + lane);
-            r.yv[k] = ld_stream(static_cast<cplx_t<T> const*>(a.ypair) + tk.pair_beg + k * kWave + lane);
+            r.pr[k] = a.pairs[tk.pair_beg + k * kWave + lane];
+            r.yv[k] = static_cast<cplx_t<T> const*>(a.ypair)[tk.pair_beg + k * kWave + lane];
         }
 #pragma unroll
-        for (int k = 0; k < kExtChunks; ++k) r.ex[k] = ld_stream(a.ext + tk.ext_beg + k * kWave + lane);
-        r.rq = ld_stream(a.rhs + tk.rhs_beg + lane);
-        r.vv = ld_stream(a.vx + tk.vx_beg + lane);
+        for (int k = 0; k < kExtChunks; ++k) r.ex[k] = a.ext[tk.ext_beg + k * kWave + lane];
+        r.rq = a.rhs[tk.rhs_beg + lane];
+        r.vv = a.vx[tk.vx_beg + lane];
     }
     return r;
 }
@@ -450,11 +426,12 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     C const* yv = pre.yv;
     // the Jacobian inputs: V and S in the factor type, from the network's
     // first bus (the pair list holds buses of the network)
+    constexpr int VS = sizeof(T) == 4 ? 2 : 1;  // FP32: V and S of a bus side by side
     C const* vsrc;
     C const* ssrc;
     if constexpr (sizeof(T) == 4) {
-        vsrc = ma.vf + b0;
-        ssrc = ma.sf + b0;
+        vsrc = reinterpret_cast<C const*>(ma.vsf + b0);
+        ssrc = vsrc + 1;
     } else {
         vsrc = ma.v + b0;
         ssrc = ma.s + b0;
@@ -478,14 +455,14 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
         int const li = pr[k].x & 0xffff, lj = static_cast<int>(static_cast<unsigned>(pr[k].x) >> 16);
         int const i0 = __builtin_amdgcn_readfirstlane(li);
         bool const diag = li == lj;
-        vi[k] = vsrc[pok[k] ? li : i0];
-        vj[k] = vsrc[pok[k] && !diag ? lj : i0];
-        si[k] = ssrc[pok[k] && diag ? li : i0];
+        vi[k] = vsrc[VS * (pok[k] ? li : i0)];
+        vj[k] = vsrc[VS * (pok[k] && !diag ? lj : i0)];
+        si[k] = ssrc[VS * (pok[k] && diag ? li : i0)];
     }
     T exv[kExtChunks];
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k)
-        exv[k] = ld_stream(front + (eok[k] ? lidx(ex[k]) : __builtin_amdgcn_readfirstlane(lidx(ex[k]))));
+        exv[k] = front[eok[k] ? lidx(ex[k]) : __builtin_amdgcn_readfirstlane(lidx(ex[k]))];
     T const vxv = vec[vok ? lidx(vv) : __builtin_amdgcn_readfirstlane(lidx(vv))];
     T const* fe = static_cast<T const*>(a.fe) + r0;  // the network's equations
     T const rv = fe[rok ? lidx(pre.rq) : __builtin_amdgcn_readfirstlane(lidx(pre.rq))];
@@ -519,20 +496,20 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     // the step's first element)
     for (int q0 = tk.pair_beg + kPairChunks * kWave; q0 < tk.pair_end; q0 += 2 * kWave) {
         int const q = q0 + lane;
-        int2 const p0 = ld_stream(a.pairs + q), p1 = ld_stream(a.pairs + q + kWave);
-        C const y0 = ld_stream(ypair + q), y1 = ld_stream(ypair + q + kWave);
+        int2 const p0 = a.pairs[q], p1 = a.pairs[q + kWave];
+        C const y0 = ypair[q], y1 = ypair[q + kWave];
         bool const ok0 = q < tk.pair_end, ok1 = q + kWave < tk.pair_end;
         int const f = __builtin_amdgcn_readfirstlane(p0.x & 0xffff);
         int const i0 = ok0 ? p0.x & 0xffff : f, j0 = ok0 ? static_cast<int>(static_cast<unsigned>(p0.x) >> 16) : f;
         int const i1 = ok1 ? p1.x & 0xffff : f, j1 = ok1 ? static_cast<int>(static_cast<unsigned>(p1.x) >> 16) : f;
-        C const vi0 = vsrc[i0], vj0 = vsrc[j0], s0 = ssrc[i0];
-        C const vi1 = vsrc[i1], vj1 = vsrc[j1], s1 = ssrc[i1];
+        C const vi0 = vsrc[VS * i0], vj0 = vsrc[VS * j0], s0 = ssrc[VS * i0];
+        C const vi1 = vsrc[VS * i1], vj1 = vsrc[VS * j1], s1 = ssrc[VS * i1];
         put_pair(ok0, p0, y0, vi0, vj0, s0);
         put_pair(ok1, p1, y1, vi1, vj1, s1);
     }
     if (rok) pool[lpos(pre.rq)] = rv;
     for (int q = tk.rhs_beg + kWave + lane; q < tk.rhs_end; q += kWave) {
-        unsigned const e = ld_stream(a.rhs + q);
+        unsigned const e = a.rhs[q];
         pool[lpos(e)] = fe[lidx(e)];
     }
     wave_sync();
@@ -545,10 +522,10 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
         unsigned e[4];
         T x[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) e[u] = ld_stream(a.ext + q + u * kWave);
+        for (int u = 0; u < 4; ++u) e[u] = a.ext[q + u * kWave];
         int const f = __builtin_amdgcn_readfirstlane(lidx(e[0]));
 #pragma unroll
-        for (int u = 0; u < 4; ++u) x[u] = ld_stream(front + (q + u * kWave < tk.ext_end ? lidx(e[u]) : f));
+        for (int u = 0; u < 4; ++u) x[u] = front[q + u * kWave < tk.ext_end ? lidx(e[u]) : f];
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (q + u * kWave < tk.ext_end) lds_add(pool + lpos(e[u]), x[u]);
@@ -556,7 +533,7 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     if (vok) lds_add(pool + lpos(vv), vxv);
     for (int q0 = tk.vx_beg + kWave; q0 < tk.vx_end; q0 += 2 * kWave) {
         int const q = q0 + lane;
-        unsigned const e0 = ld_stream(a.vx + q), e1 = ld_stream(a.vx + q + kWave);
+        unsigned const e0 = a.vx[q], e1 = a.vx[q + kWave];
         bool const ok0 = q < tk.vx_end, ok1 = q + kWave < tk.vx_end;
         int const f = __builtin_amdgcn_readfirstlane(lidx(e0));
         T const x0 = vec[ok0 ? lidx(e0) : f], x1 = vec[ok1 ? lidx(e1) : f];
@@ -662,7 +639,7 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
         for (int r = 0; r < MM; ++r) {
             if (r >= p_max) continue;  // uniform
             bool const use = r < p && t < m;
-            ucol[r] = ld_stream(front + (use ? pre.gf.x + r * m + t : g0));
+            ucol[r] = front[use ? pre.gf.x + r * m + t : g0];
         }
     }
     T* F = pool + gi * rows * ld;
@@ -670,7 +647,7 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
     wave_sync();  // the previous task of this wavefront has finished with the region
     if (sok) pool[lpos(so)] = sv;
     for (int q = tk.sol_beg + kWave + lane; q < tk.sol_end; q += kWave) {
-        unsigned const e = ld_stream(a.sol + q);
+        unsigned const e = a.sol[q];
         pool[lpos(e)] = xsol[lidx(e)];
     }
     if (t < p) w[t] = wv;

@@ -149,9 +149,9 @@ struct MisArgs {
     // of each network's first equation only (the convergence test's first row)
     void* fe;
     // grid schedule with FP32 factors: FP32 copies of V and S, the inputs of
-    // the FP32 Jacobian blocks, written by the mismatch pass
-    float2* vf;
-    float2* sf;
+    // the FP32 Jacobian blocks, written by the mismatch pass; per bus V then
+    // S (16 bytes), so a diagonal block's V and S share a cache line
+    float4* vsf;
 };
 
 __device__ inline unsigned long long dbits(double x) { return __double_as_longlong(x); }
