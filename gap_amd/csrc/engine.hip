@@ -290,6 +290,11 @@ struct GridItems {
     std::vector<int> const& bus_off;  // per instance, n_inst + 1 entries
     std::vector<int> const& var_off;  // per instance first Newton equation / variable, n_inst + 1 entries
     long long y_nnz;                  // Y-bus entries of the plan
+    // the grid schedule's own bus and equation order (grid_bus_order)
+    std::vector<int> const& bus_new;  // per bus: its local index in that order
+    std::vector<int> const& eq_new;   // per Newton equation: its local index in that order
+    std::vector<int> const& var_bus;  // per Newton variable: its bus, as a global index in that order
+    std::vector<int> const& slot_new; // per Y slot: its slot in that order
 };
 
 // The grid schedule's pair list holds bus indices local to their network in
@@ -374,6 +379,20 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
             }
         }
     d.fz_ld4 = m4 + 1;
+    // the fronts of a bucket by their first pivot's bus in the schedule's bus
+    // order, so a task's fronts gather nearby buses (a level's fronts are
+    // independent: their order changes no arithmetic)
+    {
+        auto key = [&](int n) {
+            FrontMeta const& fm = meta[n];
+            if (fm.p <= 0) return 0x7fffffff;
+            int const v = slot_col[fm.slot_beg];
+            if (v < 0 || v >= n_index) throw Error(GAPHIP_E_ARG, "pivot variable out of range");
+            return it.var_bus[v];
+        };
+        for (auto& bc : byc)
+            for (auto& F : bc) std::stable_sort(F.begin(), F.end(), [&](int x, int y) { return key(x) < key(y); });
+    }
     // Tasks of each (instance, level): classes from the largest down, and the
     // free lane groups of a class's last task take fronts of the smaller
     // classes (a front fits a larger class's group).  A task costs about the
@@ -445,6 +464,7 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     int pool = 1;
     for (int g = 0; g < G; ++g) {
       netbase[g].z = static_cast<int>(rhs.size());
+      netbase[g].w = it.var_off[g] < it.var_off[g + 1] ? it.eq_new[it.var_off[g]] : 0;
       for (int e = lvl_beg[g]; e < lvl_beg[g + 1]; ++e) {
         auto const& lv = per[e];
         for (int2 const& tr : lv) {
@@ -473,19 +493,25 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
                         if ((mask >> e & 1) && !in_tile(r0 + (e >> 1), c0 + (e & 1)))
                             throw Error(GAPHIP_E_ARG, "Jacobian pair outside its front");
                     int const b0 = it.bus_off[fm.inst];
-                    int const li = it.it_i[t] - b0, lj = it.it_j[t] - b0;
+                    if (it.it_i[t] < b0 || it.it_j[t] < b0 || it.it_i[t] >= it.bus_off[fm.inst + 1] ||
+                        it.it_j[t] >= it.bus_off[fm.inst + 1])
+                        throw Error(GAPHIP_E_ARG, "pair bus outside its network");
+                    int const li = it.bus_new[it.it_i[t]], lj = it.bus_new[it.it_j[t]];
                     if (li < 0 || lj < 0 || li > kGridMaxBuses || lj > kGridMaxBuses || li >= it.bus_off[fm.inst + 1] - b0 ||
                         lj >= it.bus_off[fm.inst + 1] - b0)
                         throw Error(GAPHIP_E_ARG, "pair bus outside its network");
                     pairs.push_back(make_int2(li | (lj << 16), (tile + r0 * ld + c0) | (mask << 16)));
-                    pair_y.push_back(it.it_y[t]);
+                    if (it.it_y[t] < 0 || it.it_y[t] >= it.y_nnz) throw Error(GAPHIP_E_ARG, "pair out of range");
+                    pair_y.push_back(it.slot_new[it.it_y[t]]);
                 }
                 fpos[n] = static_cast<int>(rhs.size());
                 for (int t = 0; t < fm.p; ++t) {
                     int const v = slot_col[fm.slot_beg + t];
                     if (v < 0 || v >= n_index || varpos[v] >= 0) throw Error(GAPHIP_E_ARG, "pivot variable out of range");
                     varpos[v] = static_cast<int>(rhs.size());
-                    rhs.push_back(make_int2(slot_row[fm.slot_beg + t] - it.var_off[g], wpos + t));
+                    int const r = slot_row[fm.slot_beg + t];
+                    if (r < it.var_off[g] || r >= it.var_off[g + 1]) throw Error(GAPHIP_E_ARG, "pivot equation out of range");
+                    rhs.push_back(make_int2(it.eq_new[r], wpos + t));
                 }
                 // (variable index here; its pivot's list position once every task is built)
                 for (int t = fm.p; t < fm.m; ++t) sol.push_back(make_int2(slot_col[fm.slot_beg + t], wpos + t));
@@ -1113,6 +1139,15 @@ struct gaphip_plan {
     gaphip::DevFronts fr;
     gaphip::DBuf<int> bus_off_d;  // per-instance bus offsets for the grid schedule
     gaphip::DBuf<int2> bpos;      // grid schedule: per bus, list positions of its angle / magnitude variables (-1: none)
+    // grid schedule: its own bus order (per network a depth-first preorder of
+    // the Y-bus graph): Y rows, slot of each plan slot, equation indices, flat
+    // start, Y contribution lists, the plan's bus of each, and V, S, S_spec
+    // and per-unit Y in it.  Its solves assemble only that Y; y_pu follows
+    // from it when something reads it (sync_plan_y)
+    gaphip::DBuf<int> g_rp, g_ci, g_slot, g_peq, g_qeq, g_old, g_seg, g_code, g_slot_inst;
+    gaphip::DBuf<double> g_flat;
+    gaphip::DBuf<double2> g_v, g_s, g_sspec, g_y;
+    std::vector<int> g_old_h;
     gaphip::DBuf<int> var_off_d;  // per-instance first Newton equation (n_inst + 1)
     int schedule = GAPHIP_SCHED_AUTO;
     gaphip_tuning tuning{2, 8, -1, 1, gaphip::kBigOuter, gaphip::kTwoLevelMin};
@@ -1132,6 +1167,7 @@ struct gaphip_plan {
     bool inputs_changed = false;      // injections, branch data or Y-bus changed since the last solve
     bool branches_changed = false;    // branch r / x or Y-bus changed since the last solve
     bool y_assembled = false;  // y_pu holds the Y-bus of the last solve's base
+    mutable bool y_from_grid = false;  // ... once copied from the grid schedule's g_y
     std::vector<std::vector<int>> br_slot_ptr, br_slot;  // per instance: Y slots each branch touches (lazy)
     gaphip::DBuf<int> upd_idx, upd_slots;
     gaphip::DBuf<double> upd_r, upd_x;
@@ -1337,6 +1373,76 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
         for (int q = 0; q < 4; ++q)
             if (it_pos[4 * t + q] >= P.j_nnz) throw Error(GAPHIP_E_ARG, "Jacobian slot out of range");
     }
+    // The grid schedule's own bus order (its kernel reads V, S and Y in it):
+    // per network the depth-first preorder of its Y-bus graph from its first
+    // bus, neighbours in row order, further components from their first bus.
+    // A bus's row neighbours then mostly lie in the same or adjacent lines of
+    // V.  Its equations are numbered along that order (P before Q of a bus).
+    // Rows keep their entries in the plan's order, so every sum is the same.
+    std::vector<int> g_old(P.n_bus), g_bus_new(P.n_bus), g_eq_new(std::max<long long>(P.n_var, 1), -1),
+        g_var_bus(std::max<long long>(P.n_var, 1), -1);
+    for (int g = 0; g < n_inst; ++g) {
+        int const b0 = P.bus_off[g], n = P.bus_off[g + 1] - b0, v0 = P.var_off[g];
+        std::vector<char> seen(n, 0);
+        std::vector<int> st;
+        int k = 0;
+        for (int r = 0; r < n; ++r) {
+            if (seen[r]) continue;
+            st.push_back(r);
+            while (!st.empty()) {
+                int const u = st.back();
+                st.pop_back();
+                if (seen[u]) continue;
+                seen[u] = 1;
+                g_old[b0 + k] = b0 + u;
+                g_bus_new[b0 + u] = k++;
+                for (int q = y_rp[b0 + u + 1] - 1; q >= y_rp[b0 + u]; --q) {
+                    int const c = y_ci[q] - b0;
+                    if (c < 0 || c >= n) throw Error(GAPHIP_E_ARG, "Y column outside its network");
+                    if (!seen[c]) st.push_back(c);
+                }
+            }
+        }
+        int e = v0;
+        for (int kk = 0; kk < n; ++kk) {
+            int const c = g_old[b0 + kk];
+            for (int const eq : {peq[c], qeq[c]})
+                if (eq >= 0) {
+                    if (eq < v0 || eq >= P.var_off[g + 1] || g_eq_new[eq] >= 0) throw Error(GAPHIP_E_ARG, "equation of two buses");
+                    g_eq_new[eq] = e++ - v0;
+                }
+            for (int const var : {ang[c], mag[c]})
+                if (var >= 0) g_var_bus[var] = b0 + kk;
+        }
+        if (e != P.var_off[g + 1]) throw Error(GAPHIP_E_ARG, "equation without a bus");
+    }
+    // the mismatch inputs in that order: Y rows (entries, and each slot's
+    // contributions, in the plan's order; columns renumbered), equation
+    // indices, flat start; g_slot maps a plan slot to the schedule's
+    std::vector<int> grp(P.n_bus + 1), gci(P.y_nnz + kRowPad, 0), gslot(std::max<long long>(P.y_nnz, 1)),
+        gpeq(P.n_bus), gqeq(P.n_bus), gseg(1, 0), gcode, gslot_inst(std::max<long long>(P.y_nnz, 1));
+    std::vector<double> gflat(P.n_bus);
+    {
+        gseg.reserve(P.y_nnz + 1);
+        gcode.reserve(y_code.size());
+        int q2 = 0;
+        for (long long k = 0; k < P.n_bus; ++k) {
+            int const i = g_old[k], g = bus_inst[i], b0 = P.bus_off[g], v0 = P.var_off[g];
+            grp[k] = q2;
+            for (int q = y_rp[i]; q < y_rp[i + 1]; ++q) {
+                gci[q2] = b0 + g_bus_new[y_ci[q]];
+                gslot_inst[q2] = y_slot_inst[q];
+                gcode.insert(gcode.end(), y_code.begin() + y_seg[q], y_code.begin() + y_seg[q + 1]);
+                gseg.push_back(static_cast<int>(gcode.size()));
+                gslot[q] = q2++;
+            }
+            gpeq[k] = peq[i] >= 0 ? v0 + g_eq_new[peq[i]] : -1;
+            gqeq[k] = qeq[i] >= 0 ? v0 + g_eq_new[qeq[i]] : -1;
+            gflat[k] = flat[i];
+        }
+        grp[P.n_bus] = q2;
+        if (q2 != P.y_nnz || gcode.size() != y_code.size()) throw Error(GAPHIP_E_ARG, "grid Y rows");
+    }
     P.bus_inst.upload(bus_inst, s);
     P.ang.upload(ang, s);
     P.mag.upload(mag, s);
@@ -1404,7 +1510,7 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
         int bus = P.node_bus[g][key >> 1];
         return P.var_off[g] + ((key & 1) ? P.np[g].mag[bus] : P.np[g].ang[bus]);
     };
-    GridItems items{it_i, it_j, it_y, it_pos, P.bus_off, P.var_off, P.y_nnz};
+    GridItems items{it_i, it_j, it_y, it_pos, P.bus_off, P.var_off, P.y_nnz, g_bus_new, g_eq_new, g_var_bus, gslot};
     pack_fronts(plans, ent_base, key_row, key_col, P.n_var, P.j_nnz, P.fr, s, &items);
     P.bus_off_d.upload(P.bus_off, s);
     P.var_off_d.upload(P.var_off, s);
@@ -1421,15 +1527,33 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
             if (!seen[e]) throw Error(GAPHIP_E_ARG, "equation without a pivot in the grid schedule");
         std::vector<int>().swap(P.fr.fz_rhs_eq);
         // the grid update gathers each bus's corrections at its variables' list positions
+        // (per bus in the schedule's order)
         auto const& vp = P.fr.fz_varpos;
         std::vector<int2> bpos(std::max<long long>(P.n_bus, 1), make_int2(-1, -1));
-        for (long long i = 0; i < P.n_bus; ++i) {
+        for (long long k = 0; k < P.n_bus; ++k) {
+            int const i = g_old[k];
             int const a = ang[i] >= 0 ? vp[ang[i]] : -1, m = mag[i] >= 0 ? vp[mag[i]] : -1;
             if ((ang[i] >= 0 && a < 0) || (mag[i] >= 0 && m < 0))
                 throw Error(GAPHIP_E_ARG, "variable without a pivot in the grid schedule");
-            bpos[i] = make_int2(a, m);
+            bpos[k] = make_int2(a, m);
         }
         P.bpos.upload(bpos, s);
+        P.g_rp.upload(grp, s);
+        P.g_ci.upload(gci, s);
+        P.g_slot.upload(gslot, s);
+        P.g_peq.upload(gpeq, s);
+        P.g_qeq.upload(gqeq, s);
+        P.g_flat.upload(gflat, s);
+        P.g_seg.upload(gseg, s);
+        P.g_code.upload(gcode, s);
+        P.g_slot_inst.upload(gslot_inst, s);
+        P.g_old.upload(g_old, s);
+        P.g_old_h = g_old;
+        P.g_v.alloc(P.n_bus);
+        P.g_s.alloc(P.n_bus);
+        P.g_sspec.alloc(P.n_bus);
+        P.g_y.alloc(P.y_nnz + kRowPad);
+        HIPCHK(hipMemsetAsync(P.g_y.p + P.y_nnz, 0, sizeof(double2) * kRowPad, s));
         HIPCHK(hipStreamSynchronize(s));
         std::vector<int>().swap(P.fr.fz_varpos);
     }
@@ -1504,14 +1628,21 @@ void upload_sspec(gaphip_plan& P, double base) {
     }
     std::vector<double2> const& c = sp;
     HIPCHK(hipMemcpyAsync(P.sspec.p, c.data(), c.size() * sizeof(double2), hipMemcpyHostToDevice, P.stream));
+    std::vector<double2> cg;
+    if (P.g_sspec.p) {
+        cg.resize(sp.size());
+        for (size_t k = 0; k < sp.size(); ++k) cg[k] = sp[P.g_old_h[k]];
+        HIPCHK(hipMemcpyAsync(P.g_sspec.p, cg.data(), cg.size() * sizeof(double2), hipMemcpyHostToDevice, P.stream));
+    }
     HIPCHK(hipMemcpyAsync(P.zb.p, zb.data(), zb.size() * sizeof(double), hipMemcpyHostToDevice, P.stream));
     HIPCHK(hipStreamSynchronize(P.stream));
     P.sspec_dirty = false;
     P.sspec_base = base;
 }
 
-// Overwrite the assembled per-unit Y of instances that were given a Y-bus.
-void apply_given_y(gaphip_plan& P, double base) {
+// Overwrite the assembled per-unit Y of instances that were given a Y-bus
+// (grid: in the grid schedule's copy instead of the plan's).
+void apply_given_y(gaphip_plan& P, double base, bool grid = false) {
     if (P.y_given_mode.empty()) return;
     for (int g = 0; g < P.n_inst; ++g) {
         int mode = P.y_given_mode[g];
@@ -1521,8 +1652,21 @@ void apply_given_y(gaphip_plan& P, double base) {
         int b = P.y_off[g], e = P.y_off[g + 1];
         if (e > b)
             hipLaunchKernelGGL(k_yscale, dim3(blocks_for(e - b, 256)), dim3(256), 0, P.stream, b, e, P.y_given.p,
-                               scale, P.y_pu.p);
+                               scale, grid ? nullptr : P.y_pu.p, grid ? P.g_slot.p : nullptr,
+                               grid ? P.g_y.p : nullptr);
     }
+}
+
+// The plan's Y (y_pu) after a grid-schedule solve, which assembled only the
+// schedule's copy: gathered from it the first time something reads it.
+void sync_plan_y(gaphip_plan const& P) {
+    if (!P.y_from_grid) return;
+    if (P.y_nnz > 0)
+        hipLaunchKernelGGL(k_ygather, dim3(blocks_for(P.y_nnz, 256)), dim3(256), 0, P.stream, P.y_nnz, P.g_slot.p,
+                           P.g_y.p, P.y_pu.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(P.stream));
+    P.y_from_grid = false;
 }
 
 // Launch helper that optionally brackets a launch with timing events.
@@ -1617,6 +1761,7 @@ void run_solve(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front, T* v
     launch(P, K_YBUS, 48.0 * Z, [&] { hipLaunchKernelGGL(k_ybus, dim3(blocks_for(Z, 256)), dim3(256), 0, s, ya); });
     apply_given_y(P, o.base_power);
     P.y_assembled = true;
+    P.y_from_grid = false;
     FlatArgs fa{static_cast<int>(N), P.flat.p, P.v.p};
     launch(P, K_FLAT, 24.0 * N, [&] { hipLaunchKernelGGL(k_flat, dim3(blocks_for(N, 256)), dim3(256), 0, s, fa); });
 
@@ -1719,18 +1864,22 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
     if (P.prof.on) P.prof.reset();
     long long const N = P.n_bus, Z = P.y_nnz;
 
-    YbusArgs ya{static_cast<int>(Z), P.y_seg.p, P.y_code.p, P.y_slot_inst.p, P.br_r.p, P.br_x.p, P.br_g.p,
-                P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.y_pu.p, nullptr};
+    // Y and the flat start in the grid schedule's own bus order (its kernel
+    // writes V back in the plan's order; with no iteration V is the flat start)
+    YbusArgs ya{static_cast<int>(Z), P.g_seg.p, P.g_code.p, P.g_slot_inst.p, P.br_r.p, P.br_x.p, P.br_g.p,
+                P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.g_y.p, nullptr};
     launch(P, K_YBUS, 48.0 * Z, [&] { hipLaunchKernelGGL(k_ybus, dim3(blocks_for(Z, 256)), dim3(256), 0, s, ya); });
-    apply_given_y(P, o.base_power);
+    apply_given_y(P, o.base_power, true);
     P.y_assembled = true;
-    FlatArgs fa{static_cast<int>(N), P.flat.p, P.v.p};
+    P.y_from_grid = true;
+    FlatArgs fa = max_it > 0 ? FlatArgs{static_cast<int>(N), P.g_flat.p, P.g_v.p}
+                             : FlatArgs{static_cast<int>(N), P.flat.p, P.v.p};
     launch(P, K_FLAT, 24.0 * N, [&] { hipLaunchKernelGGL(k_flat, dim3(blocks_for(N, 256)), dim3(256), 0, s, fa); });
 
     if (max_it > 0) {
         DevFronts const& d = P.fr;
         hipLaunchKernelGGL(k_gather_ypair<T>, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
-                           static_cast<int>(d.fz_npairs), d.fz_pair_y.p, P.y_pu.p,
+                           static_cast<int>(d.fz_npairs), d.fz_pair_y.p, P.g_y.p,
                            reinterpret_cast<cplx_t<T>*>(d.fz_ypair.p));
         if (sizeof(T) == 4 && !P.vsf.p) {
             P.vsf.alloc(N);
@@ -1738,10 +1887,10 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
         FusedArgs za{G, max_it, o.tolerance, 0, d.fz_order.p, P.bus_off_d.p, P.var_off_d.p, d.fz_lvl_beg.p, d.fz_lvl_ptr.p,
                      d.fz_tasks.p, d.fz_gfront.p, d.fz_pairs.p, d.fz_ypair.p, d.fz_fe.p, d.fz_rhs.p, d.fz_ext.p, d.fz_vx.p, d.fz_sol.p, d.fz_netbase.p, d.fz_pool, d.fz_ld4,
                      P.maxf.p, P.done.p, P.iters.p, P.lu_fail.p, P.final_mis.p, d.fz_xs.p};
-        MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
-                   P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p, d.fz_fe.p, P.vsf.p};
-        UpdArgs ua{static_cast<int>(N), P.bus_inst.p, P.done.p, P.lu_fail.p, P.ang.p, P.mag.p, P.dx.p, P.v.p,
-                   P.bpos.p, d.fz_xs.p};
+        MisArgs ma{static_cast<int>(N), P.g_rp.p, P.g_ci.p, P.g_y.p, P.g_v.p, P.g_sspec.p, P.g_peq.p, P.g_qeq.p,
+                   nullptr, P.done.p, P.g_s.p, nullptr, P.f.p, P.maxf.p, d.fz_fe.p, P.vsf.p};
+        UpdArgs ua{static_cast<int>(N), nullptr, P.done.p, P.lu_fail.p, nullptr, nullptr, nullptr, P.g_v.p,
+                   P.bpos.p, d.fz_xs.p, P.g_old.p, P.v.p};
         FrontArgs fra = d.args(nullptr, P.lu_fail.p);
         gaphip_tuning const& tu = P.tuning;
         int nbig = 0;
@@ -1841,6 +1990,7 @@ void run_adjoint(gaphip_plan& P, double w_loss, double w_dev, double const* g_vm
         P.maxf_rows = 1;
     }
     // S, |V|, arg V at the solution, then J(x*) and its FP64 factors
+    sync_plan_y(P);
     MisArgs ma{static_cast<int>(N), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
                P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p};
     hipLaunchKernelGGL(k_mismatch, dim3(blocks_for(N, 256)), dim3(256), 0, s, ma);
@@ -1934,6 +2084,7 @@ void run_update_branches(gaphip_plan& P, int g, int n, const int* branch, const 
                        P.br_r.p, P.br_x.p);
     bool const given = !P.y_given_mode.empty() && P.y_given_mode[g];
     if (P.y_assembled && !given && !slots.empty()) {
+        sync_plan_y(P);
         P.upd_slots.upload(slots, s);
         YbusArgs ya{static_cast<int>(P.y_nnz), P.y_seg.p, P.y_code.p, P.y_slot_inst.p, P.br_r.p, P.br_x.p, P.br_g.p,
                     P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.y_pu.p, nullptr};
@@ -1955,6 +2106,7 @@ void run_get_ybus(gaphip_plan const& P, int g, int* rp, int* ci, double* val_ri,
                                                       std::to_string(nnz));
     if (rp) std::copy(Y.rp.begin(), Y.rp.end(), rp);
     if (ci) std::copy(Y.ci.begin(), Y.ci.end(), ci);
+    sync_plan_y(P);
     if (val_ri && nnz)
         HIPCHK(hipMemcpy(val_ri, P.y_pu.p + P.y_off[g], sizeof(double2) * nnz, hipMemcpyDeviceToHost));
 }
@@ -2392,6 +2544,7 @@ void evaluate_instance(gaphip_plan& P, int inst, double const* v_ri, double base
                 P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.y_pu.p, nullptr};
     hipLaunchKernelGGL(k_ybus, dim3(blocks_for(P.y_nnz, 256)), dim3(256), 0, s, ya);
     apply_given_y(P, base);
+    P.y_from_grid = false;
     MisArgs ma{static_cast<int>(P.n_bus), P.y_rp.p, P.y_ci.p, P.y_pu.p, P.v.p, P.sspec.p, P.peq.p, P.qeq.p,
                P.bus_inst.p, P.done.p, P.s.p, P.vpol.p, P.f.p, P.maxf.p};
     hipLaunchKernelGGL(k_mismatch, dim3(blocks_for(P.n_bus, 256)), dim3(256), 0, s, ma);

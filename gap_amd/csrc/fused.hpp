@@ -68,7 +68,10 @@ struct FusedArgs {
     const unsigned* ext;  // (front storage entry, LDS position)
     const unsigned* vx;   // (vector entry, LDS position)
     const unsigned* sol;  // (list position of the variable's pivot, LDS position)
-    const int4* netbase;  // per network: first front storage entry, first vector entry, first list entry
+    // per network: first front storage entry, first vector entry, first list
+    // entry, and the local index (in the schedule's numbering) of the
+    // network's first equation in the reference's numbering
+    const int4* netbase;
     int pool;             // LDS elements per wavefront
     int ld4;              // leading dimension of the 64-slot class tiles (largest such front + 1)
     unsigned long long* maxf;  // rows of per-network max |F| per iteration
@@ -155,13 +158,14 @@ static_assert(kListPad >= kPairChunks * kWave && kListPad >= kExtChunks * kWave 
 // S_i = V_i conj((Y V)_i) and the P / Q mismatch rows of buses i[0 .. B-1]
 // (those past the first only where use_rest); returns their max |F|.  Each
 // equation's -F is written in the factor type, by equation index (the factor
-// tasks gather it); the FP64 F only for the network's first equation r0,
-// the one row the convergence test reads back.  The B rows'
+// tasks gather it); the FP64 F only for the network's first equation in the
+// reference's numbering (rf in this schedule's numbering), stored at r0: the
+// one row the convergence test reads back.  The B rows'
 // loads are issued together, four entries per row and step; the sums keep the
 // row order.  B = 2 pays with FP64 factors; with FP32 factors its registers
 // would spill (the kernel is held to 128).
 template <typename T, int B>
-__device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B], bool use_rest, int r0) {
+__device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B], bool use_rest, int r0, int rf) {
 #pragma clang fp contract(off)
     double cr[B], cim[B];
     int qb[B], qe[B];
@@ -221,13 +225,13 @@ __device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B]
         }
         if (pe >= 0) {
             double const fp = sr - sp.x;
-            if (pe == r0) a.f[pe] = fp;
+            if (pe == rf) a.f[r0] = fp;
             fe[pe] = static_cast<T>(-1.0 * fp);
             m = fmax(m, fabs(fp));
         }
         if (qeq >= 0) {
             double const fq = si - sp.y;
-            if (qeq == r0) a.f[qeq] = fq;
+            if (qeq == rf) a.f[r0] = fq;
             fe[qeq] = static_cast<T>(-1.0 * fq);
             m = fmax(m, fabs(fq));
         }
@@ -862,7 +866,7 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 5 : 3)
             int ib[MB];
 #pragma unroll
             for (int b = 0; b < MB; ++b) ib[b] = min(i + b * NT, b1 - 1);
-            m = fmax(m, grid_mismatch_buses<T, MB>(ma, ib, i + (MB - 1) * NT < b1, r0));
+            m = fmax(m, grid_mismatch_buses<T, MB>(ma, ib, i + (MB - 1) * NT < b1, r0, r0 + nb.w));
         }
         m = block_max<NW>(m, red);  // its barriers also publish S and F
         if (r0 < r1) m = reference_max(m, ma.f[r0]);
@@ -886,6 +890,9 @@ __global__ void __launch_bounds__(NW* kWave, sizeof(T) == 4 ? 5 : 3)
         for (int i = b0 + tid; i < b1; i += NT) grid_update_bus<T>(ua, i, zero);
         __syncthreads();
     }
+    // V back in the plan's bus order (each thread's buses are the ones its
+    // last update wrote)
+    for (int i = b0 + tid; i < b1; i += NT) ua.v_out[ua.perm[i]] = ua.v[i];
 }
 
 }  // namespace gaphip

@@ -106,11 +106,19 @@ __global__ void __launch_bounds__(256) k_ybus_slots(YbusArgs a, int n, const int
 // Caller-supplied Y-bus: y_pu = given * scale (scale = Z_base, or 1 when the
 // values are already per unit), slots [begin, end).
 __global__ void __launch_bounds__(256) k_yscale(int begin, int end, const double2* given, double scale,
-                                                double2* y_pu) {
+                                                double2* y_pu, const int* slot_g, double2* y_g) {
     int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= end) return;
     double2 g = given[k];
-    y_pu[k] = make_double2(g.x * scale, g.y * scale);
+    double2 const y = make_double2(g.x * scale, g.y * scale);
+    if (y_pu) y_pu[k] = y;
+    if (y_g) y_g[slot_g[k]] = y;  // the grid schedule's copy (its own slot order)
+}
+
+// The plan's Y from the grid schedule's copy (slot k is its slot slot_g[k]).
+__global__ void __launch_bounds__(256) k_ygather(long long n, const int* slot_g, const double2* y_g, double2* y_pu) {
+    long long const k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) y_pu[k] = y_g[slot_g[k]];
 }
 
 // ------------------------------------------------------------ flat start
@@ -1375,6 +1383,10 @@ struct UpdArgs {
     // list order (-1: no such variable)
     const int2* bpos;
     const void* xs;
+    // grid schedule: v holds the network's buses in the schedule's own order;
+    // at the end of the solve bus i is written to v_out[perm[i]]
+    const int* perm;
+    double2* v_out;
 };
 __device__ inline void update_bus(UpdArgs const& a, int i, bool zero) {
 #pragma clang fp contract(off)
