@@ -247,6 +247,7 @@ struct DevFronts {
     DBuf<int4> fz_gfront;  // per front: storage offset, vector offset, m | p << 16, list position of its first pivot
     DBuf<int2> fz_pairs;
     DBuf<int> fz_pair_y;
+    DBuf<int> fz_slot_pair;  // per Y slot (grid order): its pair, or -1 (when no slot has two)
     DBuf<double2> fz_ypair;  // Y value of each pair, refreshed at every solve
     long long fz_npairs = 0;
     // -F of each Newton equation (factor type), written by every mismatch
@@ -655,8 +656,21 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
     d.fz_gfront.upload(gfront, s);
     d.fz_pairs.upload(pairs, s);
     d.fz_pair_y.upload(pair_y, s);
+    // the Y assembly writes each pair's value itself where no slot feeds two
+    // pairs (the list padding past the pairs is not a pair)
+    std::vector<int> slot_pair(std::max<long long>(it.y_nnz, 1), -1);
+    {
+        bool one = true;
+        for (size_t q = 0; q + kListPad < pair_y.size() && one; ++q) {
+            if (slot_pair[pair_y[q]] >= 0) one = false;
+            slot_pair[pair_y[q]] = static_cast<int>(q);
+        }
+        if (one) d.fz_slot_pair.upload(slot_pair, s);
+        else d.fz_slot_pair.release();
+    }
     d.fz_npairs = static_cast<long long>(pairs.size());
     d.fz_ypair.alloc(d.fz_npairs);
+    HIPCHK(hipMemsetAsync(d.fz_ypair.p, 0, sizeof(double2) * d.fz_npairs, s));  // (the padding stays zero)
     d.fz_fe.alloc(rhs.size());  // one per Newton equation; doubles: room for either factor type
     HIPCHK(hipMemsetAsync(d.fz_fe.p, 0, sizeof(double) * rhs.size(), s));
     d.fz_xs.alloc(rhs.size() + kVecPad);
@@ -1144,7 +1158,11 @@ struct gaphip_plan {
     // start, Y contribution lists, the plan's bus of each, and V, S, S_spec
     // and per-unit Y in it.  Its solves assemble only that Y; y_pu follows
     // from it when something reads it (sync_plan_y)
-    gaphip::DBuf<int> g_rp, g_ci, g_slot, g_peq, g_qeq, g_old, g_seg, g_code, g_slot_inst;
+    gaphip::DBuf<int> g_rp, g_ci, g_slot, g_peq, g_qeq, g_old, g_seg, g_code, g_slot_inst, g_br_src, g_app_src;
+    gaphip::DBuf<double4> g_brp;  // branch (r, x, g, b) records of the grid Y assembly (repacked when dirty)
+    gaphip::DBuf<double2> g_app;  // shunt (g, b) records
+    int g_n_brp = 0, g_n_app = 0;
+    bool g_par_dirty = true;
     gaphip::DBuf<double> g_flat;
     gaphip::DBuf<double2> g_v, g_s, g_sspec, g_y;
     std::vector<int> g_old_h;
@@ -1422,9 +1440,13 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
     std::vector<int> grp(P.n_bus + 1), gci(P.y_nnz + kRowPad, 0), gslot(std::max<long long>(P.y_nnz, 1)),
         gpeq(P.n_bus), gqeq(P.n_bus), gseg(1, 0), gcode, gslot_inst(std::max<long long>(P.y_nnz, 1));
     std::vector<double> gflat(P.n_bus);
+    // (the contribution lists index packed branch / shunt records, numbered in
+    // the order the slots first use them: g_br_src / g_app_src)
+    std::vector<int> g_br_src, g_app_src;
     {
         gseg.reserve(P.y_nnz + 1);
         gcode.reserve(y_code.size());
+        std::vector<int> br_pos(std::max<long long>(P.n_br, 1), -1), app_pos(std::max<long long>(P.n_app, 1), -1);
         int q2 = 0;
         for (long long k = 0; k < P.n_bus; ++k) {
             int const i = g_old[k], g = bus_inst[i], b0 = P.bus_off[g], v0 = P.var_off[g];
@@ -1432,7 +1454,16 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
             for (int q = y_rp[i]; q < y_rp[i + 1]; ++q) {
                 gci[q2] = b0 + g_bus_new[y_ci[q]];
                 gslot_inst[q2] = y_slot_inst[q];
-                gcode.insert(gcode.end(), y_code.begin() + y_seg[q], y_code.begin() + y_seg[q + 1]);
+                for (int t = y_seg[q]; t < y_seg[q + 1]; ++t) {
+                    int const c = y_code[t], kind = c & 3, idx = c >> 2;
+                    std::vector<int>& pos = kind == kContribShunt ? app_pos : br_pos;
+                    std::vector<int>& src = kind == kContribShunt ? g_app_src : g_br_src;
+                    if (pos[idx] < 0) {
+                        pos[idx] = static_cast<int>(src.size());
+                        src.push_back(idx);
+                    }
+                    gcode.push_back((pos[idx] << 2) | kind);
+                }
                 gseg.push_back(static_cast<int>(gcode.size()));
                 gslot[q] = q2++;
             }
@@ -1546,6 +1577,13 @@ void build_plan(gaphip_plan& P, int n_inst, gaphip_net const* nets, int precisio
         P.g_flat.upload(gflat, s);
         P.g_seg.upload(gseg, s);
         P.g_code.upload(gcode, s);
+        P.g_n_brp = static_cast<int>(g_br_src.size());
+        P.g_n_app = static_cast<int>(g_app_src.size());
+        P.g_br_src.upload(g_br_src, s);
+        P.g_app_src.upload(g_app_src, s);
+        P.g_brp.alloc(std::max(P.g_n_brp, 1));
+        P.g_app.alloc(std::max(P.g_n_app, 1));
+        P.g_par_dirty = true;
         P.g_slot_inst.upload(gslot_inst, s);
         P.g_old.upload(g_old, s);
         P.g_old_h = g_old;
@@ -1866,9 +1904,22 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
 
     // Y and the flat start in the grid schedule's own bus order (its kernel
     // writes V back in the plan's order; with no iteration V is the flat start)
-    YbusArgs ya{static_cast<int>(Z), P.g_seg.p, P.g_code.p, P.g_slot_inst.p, P.br_r.p, P.br_x.p, P.br_g.p,
-                P.br_b.p, P.app_g.p, P.app_b.p, P.zb.p, P.g_y.p, nullptr};
-    launch(P, K_YBUS, 48.0 * Z, [&] { hipLaunchKernelGGL(k_ybus, dim3(blocks_for(Z, 256)), dim3(256), 0, s, ya); });
+    DevFronts const& d = P.fr;
+    if (P.g_par_dirty) {
+        YbusPackArgs pa{P.g_n_brp, P.g_n_app, P.g_br_src.p, P.g_app_src.p, P.br_r.p, P.br_x.p, P.br_g.p, P.br_b.p,
+                        P.app_g.p, P.app_b.p, P.g_brp.p, P.g_app.p};
+        int const n = std::max(P.g_n_brp, P.g_n_app);
+        if (n > 0) hipLaunchKernelGGL(k_ybus_pack, dim3(blocks_for(n, 256)), dim3(256), 0, s, pa);
+        P.g_par_dirty = false;
+    }
+    // the pairs' Y values come with the assembly unless an instance's Y is
+    // given (then gathered after it is applied)
+    bool const given = std::any_of(P.y_given_mode.begin(), P.y_given_mode.end(), [](int m) { return m != 0; });
+    bool const fused_pairs = max_it > 0 && d.fz_slot_pair.p && !given;
+    YbusGridArgs ya{static_cast<int>(Z), P.g_seg.p, P.g_code.p, P.g_slot_inst.p, P.g_brp.p, P.g_app.p, P.zb.p,
+                    P.g_y.p, fused_pairs ? d.fz_slot_pair.p : nullptr, d.fz_ypair.p};
+    launch(P, K_YBUS, 48.0 * Z,
+           [&] { hipLaunchKernelGGL(k_ybus_grid<T>, dim3(blocks_for(Z, 256)), dim3(256), 0, s, ya); });
     apply_given_y(P, o.base_power, true);
     P.y_assembled = true;
     P.y_from_grid = true;
@@ -1877,10 +1928,10 @@ void run_solve_fused(gaphip_plan& P, gaphip_solve_opts const& o, T* jv, T* front
     launch(P, K_FLAT, 24.0 * N, [&] { hipLaunchKernelGGL(k_flat, dim3(blocks_for(N, 256)), dim3(256), 0, s, fa); });
 
     if (max_it > 0) {
-        DevFronts const& d = P.fr;
-        hipLaunchKernelGGL(k_gather_ypair<T>, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
-                           static_cast<int>(d.fz_npairs), d.fz_pair_y.p, P.g_y.p,
-                           reinterpret_cast<cplx_t<T>*>(d.fz_ypair.p));
+        if (!fused_pairs)
+            hipLaunchKernelGGL(k_gather_ypair<T>, dim3(blocks_for(d.fz_npairs, 256)), dim3(256), 0, s,
+                               static_cast<int>(d.fz_npairs), d.fz_pair_y.p, P.g_y.p,
+                               reinterpret_cast<cplx_t<T>*>(d.fz_ypair.p));
         if (sizeof(T) == 4 && !P.vsf.p) {
             P.vsf.alloc(N);
         }
@@ -2046,7 +2097,7 @@ void run_update_branches(gaphip_plan& P, int g, int n, const int* branch, const 
     int const m = N.n_branch();
     for (int t = 0; t < n; ++t)
         if (branch[t] < 0 || branch[t] >= m) throw Error(GAPHIP_E_ARG, "branch index out of range");
-    P.inputs_changed = P.branches_changed = true;
+    P.inputs_changed = P.branches_changed = P.g_par_dirty = true;
     if (P.br_slot_ptr.empty()) {
         P.br_slot_ptr.resize(P.n_inst);
         P.br_slot.resize(P.n_inst);
@@ -2266,7 +2317,7 @@ int gaphip_plan_set_branch_rx(gaphip_plan* P, int inst, const double* r, const d
         int const m = N.n_branch();
         N.br_r.assign(r, r + m);
         N.br_x.assign(x, x + m);
-        P->inputs_changed = P->branches_changed = true;
+        P->inputs_changed = P->branches_changed = P->g_par_dirty = true;
         if (m) {
             HIPCHK(hipMemcpyAsync(P->br_r.p + P->br_off[inst], r, sizeof(double) * m, hipMemcpyHostToDevice,
                                   P->stream));
@@ -2282,7 +2333,7 @@ int gaphip_plan_set_ybus(gaphip_plan* P, int inst, const int* rp, const int* ci,
     GAPHIP_GUARD({
         if (!P || inst < 0 || inst >= P->n_inst) throw Error(GAPHIP_E_ARG, "bad argument");
         if (P->y_given_mode.empty()) P->y_given_mode.assign(P->n_inst, 0);
-        P->inputs_changed = P->branches_changed = true;
+        P->inputs_changed = P->branches_changed = P->g_par_dirty = true;
         if (!rp) {
             P->y_given_mode[inst] = 0;
             return GAPHIP_OK;

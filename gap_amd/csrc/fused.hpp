@@ -733,6 +733,94 @@ __device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArg
     }
 }
 
+// The grid schedule's Y assembly, in its own slot order: the same
+// per-slot arithmetic and contribution order as ybus_slot, with the branch
+// (r, x, g, b) and shunt (g, b) parameters packed in the order the slots
+// first use them, so a slot's reads are one record each and consecutive
+// slots read consecutive records.  Where slot_pair is given, the slot's
+// value is also written, in the factor type, at its pair (at most one).
+struct YbusGridArgs {
+    int nnz;
+    const int* seg;
+    const int* code;  // index into brp (branches) or app (shunts) << 2 | kind
+    const int* slot_inst;
+    const double4* brp;
+    const double2* app;
+    const double* zb;
+    double2* y;
+    const int* slot_pair;
+    void* ypair;
+};
+struct YbusPackArgs {
+    int n_br, n_app;
+    const int* br_src;  // per record: its branch
+    const int* app_src; // per record: its shunt
+    const double* br_r;
+    const double* br_x;
+    const double* br_g;
+    const double* br_b;
+    const double* app_g;
+    const double* app_b;
+    double4* brp;
+    double2* app;
+};
+__global__ void __launch_bounds__(256) k_ybus_pack(YbusPackArgs a) {
+    int const k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < a.n_br) {
+        int const b = a.br_src[k];
+        a.brp[k] = make_double4(a.br_r[b], a.br_x[b], a.br_g[b], a.br_b[b]);
+    }
+    if (k < a.n_app) {
+        int const s = a.app_src[k];
+        a.app[k] = make_double2(a.app_g[s], a.app_b[s]);
+    }
+}
+template <typename T>
+__global__ void __launch_bounds__(256) k_ybus_grid(YbusGridArgs a) {
+#pragma clang fp contract(off)
+    int const k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nnz) return;
+    double re = 0.0, im = 0.0;
+    for (int s = a.seg[k]; s < a.seg[k + 1]; ++s) {
+        int const c = a.code[s];
+        int const kind = c & 3, idx = c >> 2;
+        if (kind == 0) {
+            double2 const sh = a.app[idx];
+            re += sh.x;
+            im += sh.y;
+        } else {
+            double4 const p = a.brp[idx];
+            double const r = p.x, x = p.y;
+            double const z2 = r * r + x * x;
+            double yr = 0.0, yi = 0.0;
+            if (z2 > 1e-12) {
+                yr = r / z2;
+                yi = -x / z2;
+            }
+            if (kind == 2) {
+                re = -yr;
+                im = -yi;
+            } else {
+                double const bar = yr + p.z, bai = yi + 0.0;
+                double const tr = bar + 0.0, ti = bai + p.w / 2.0;
+                re += tr;
+                im += ti;
+            }
+        }
+    }
+    double const zb = a.zb[a.slot_inst[k]];
+    double2 const y = make_double2(re * zb, im * zb);
+    a.y[k] = y;
+    if (a.slot_pair) {
+        int const q = a.slot_pair[k];
+        if (q >= 0) {
+            cplx_t<T>* yp = static_cast<cplx_t<T>*>(a.ypair);
+            yp[q].x = static_cast<T>(y.x);
+            yp[q].y = static_cast<T>(y.y);
+        }
+    }
+}
+
 // Per-unit Y of every pair in pair order, in the factor type, once per solve.
 template <typename T>
 __global__ void __launch_bounds__(256) k_gather_ypair(int n, const int* pair_y, const double2* y, cplx_t<T>* ypair) {
