@@ -254,6 +254,41 @@ def test_grid_schedule_matches_levels(device, precision):
     assert all(np.array_equal(a, b) for a, b in zip(rg["v"], rg2["v"]))
 
 
+def test_grid_bus_order_is_internal(device):
+    """The grid schedule works on its own bus order (per network a depth-first
+    order of the Y-bus graph) inside the plan; what a caller reads back is in
+    the plan's order: the Y-bus after a grid solve (copied from the schedule's
+    copy) bit-exact against the standalone assembly and the level schedule's,
+    branch updates applied to it, and the adjoint equal to the level
+    schedule's to rounding."""
+    nets = _batch(10, 777)
+    g = capi.Plan(nets, precision="fp64", schedule="grid")
+    lv = capi.Plan(nets, precision="fp64", schedule="levels")
+    assert g.schedule == "grid" and lv.schedule == "levels"
+    rg = g.solve(1e-8, 100, 100e6)
+    rl = lv.solve(1e-8, 100, 100e6)
+    assert rg["converged"].all() and np.array_equal(rg["iterations"], rl["iterations"])
+    for k, net in enumerate(nets):
+        assert np.abs(rg["v"][k] - rl["v"][k]).max() <= 1e-12
+        rp, ci, yv = g.ybus(k)
+        rp2, ci2, yv2 = lv.ybus(k)
+        rp3, ci3, yv3 = capi.ybus(net)
+        assert np.array_equal(rp, rp2) and np.array_equal(ci, ci2) and np.array_equal(yv, yv2)
+        assert np.array_equal(rp, rp3) and np.array_equal(ci, ci3)
+    ag, al = g.adjoint(1.0, 0.5), lv.adjoint(1.0, 0.5)
+    for key in ag:
+        assert np.allclose(ag[key], al[key], rtol=1e-9, atol=1e-12), key
+    # a branch update after the grid solve lands in the plan's Y, and the
+    # next grid solve assembles the schedule's copy from the new values
+    g.update_branches(3, [0, 2], [0.3, 0.2], [0.4, 0.45])
+    lv.update_branches(3, [0, 2], [0.3, 0.2], [0.4, 0.45])
+    assert np.array_equal(g.ybus(3)[2], lv.ybus(3)[2])
+    rg = g.solve(1e-8, 100, 100e6)
+    rl = lv.solve(1e-8, 100, 100e6)
+    assert np.array_equal(rg["iterations"], rl["iterations"])
+    assert max(np.abs(a - b).max() for a, b in zip(rg["v"], rl["v"])) <= 1e-12
+
+
 def test_grid_tuning_does_not_change_results(device):
     from gap_amd import tuning
     nets = _batch(12, 321) + [gridgen.small_mesh(4000, seed=5, n_extra=40, load_factor=0.5)]
