@@ -555,11 +555,15 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
             e.x = varpos[e.x] - netbase[g].z;
         }
     }
-    // Dependencies between the tasks of a network: the task holding the last
-    // child (factorisation) and the first parent (backward substitution) of
-    // any of a task's fronts.  Tasks are in level order and a child's level is
-    // below its parent's, so dep_f < task < dep_b, which the kernel's waits
-    // rely on to end.
+    // Dependencies between the tasks of a network, per wavefront of a sweep
+    // of kDepWaves wavefronts: for each residue r of a task position (counted
+    // from the network's first task in the factorisation, from its last in
+    // the backward substitution; the wavefront that runs it), the distance
+    // to the nearest task of residue r holding a child (factorisation) or a
+    // parent (backward) of one of the task's fronts, 0 for none.  Tasks are in
+    // level order and a child's level is below its parent's, so children come
+    // before a task and parents after it, which the kernel's waits rely on to
+    // end.
     {
         std::vector<int> task_of(nf, -1), parent(nf, -1);
         for (size_t k = 0; k < tasks.size(); ++k)
@@ -570,18 +574,34 @@ void build_grid(DevFronts& d, std::vector<LUPlan const*> const& plans, GridItems
                 if (cinfo[ci].id < 0 || cinfo[ci].id >= nf) throw Error(GAPHIP_E_ARG, "child front out of range");
                 parent[cinfo[ci].id] = n;
             }
-        for (size_t k = 0; k < tasks.size(); ++k) {
-            GridTask& tk = tasks[k];
-            tk.dep_f = -1;
-            tk.dep_b = 0x7fffffff;
-            for (int q = tk.front_beg; q < tk.front_beg + (tk.cls_cnt & 0xff); ++q) {
-                int const n = tfront[q];
-                for (int ci = meta[n].child_beg; ci < meta[n].child_end; ++ci)
-                    tk.dep_f = std::max(tk.dep_f, task_of[cinfo[ci].id]);
-                if (parent[n] >= 0) tk.dep_b = std::min(tk.dep_b, task_of[parent[n]]);
+        for (int g = 0; g < G; ++g) {
+            int const t0 = lvl_ptr[lvl_beg[g]], t1 = lvl_ptr[lvl_beg[g + 1]];
+            for (int k = t0; k < t1; ++k) {
+                GridTask& tk = tasks[k];
+                unsigned of[kDepWaves] = {}, ob[kDepWaves] = {};
+                auto note = [&](unsigned* o, int r, long long dist) {
+                    if (dist <= 0) throw Error(GAPHIP_E_ARG, "grid task dependencies out of level order");
+                    if (dist > 0xffff) throw Error(GAPHIP_E_ARG, "grid task dependency too far");
+                    o[r] = o[r] ? std::min<unsigned>(o[r], static_cast<unsigned>(dist)) : static_cast<unsigned>(dist);
+                };
+                for (int q = tk.front_beg; q < tk.front_beg + (tk.cls_cnt & 0xff); ++q) {
+                    int const n = tfront[q];
+                    for (int ci = meta[n].child_beg; ci < meta[n].child_end; ++ci) {
+                        int const kc = task_of[cinfo[ci].id];
+                        if (kc < t0 || kc >= t1) throw Error(GAPHIP_E_ARG, "child task outside its network");
+                        note(of, (kc - t0) % kDepWaves, k - kc);
+                    }
+                    if (parent[n] >= 0) {
+                        int const kp = task_of[parent[n]];
+                        if (kp < t0 || kp >= t1) throw Error(GAPHIP_E_ARG, "parent task outside its network");
+                        note(ob, (t1 - 1 - kp) % kDepWaves, kp - k);
+                    }
+                }
+                for (int h = 0; h < kDepWaves / 2; ++h) {
+                    tk.dep_f[h] = of[2 * h] | (of[2 * h + 1] << 16);
+                    tk.dep_b[h] = ob[2 * h] | (ob[2 * h + 1] << 16);
+                }
             }
-            if (tk.dep_f >= static_cast<int>(k) || tk.dep_b <= static_cast<int>(k))
-                throw Error(GAPHIP_E_ARG, "grid task dependencies out of level order");
         }
     }
     // bounds of every gathered index and LDS position

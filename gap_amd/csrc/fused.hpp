@@ -42,9 +42,15 @@ struct alignas(16) GridTask {
     int ext_beg, ext_end;
     int vx_beg, vx_end;
     int sol_beg, sol_end;
-    int dep_f;  // largest task index holding a child of one of its fronts (-1: none)
-    int dep_b;  // smallest task index holding a parent of one of its fronts (INT_MAX: none)
+    // For each residue r < kDepWaves of a task position modulo kDepWaves
+    // (the factorisation counts from the network's first task, the backward
+    // substitution from its last), 16 bits each: the distance to the nearest
+    // task of residue r that holds a child (dep_f) or a parent (dep_b) of
+    // one of this task's fronts; 0 for none.
+    unsigned dep_f[2];
+    unsigned dep_b[2];
 };
+constexpr int kDepWaves = 4;
 
 struct FusedArgs {
     int n_inst;
@@ -836,20 +842,42 @@ __global__ void __launch_bounds__(256) k_gather_ypair(int n, const int* pair_y, 
 // index of the next task wavefront w will run (its tasks are w, w + NW, ...
 // of the network's task list, in order).  Tasks are in level order, children
 // before parents, so in the factorisation every task's children have smaller
-// indices (dep_f < index) and in the backward substitution every task's
-// parents larger ones (dep_b > index); the wavefront whose next task comes
-// first (last) in the sweep always finds its dependencies done, so every wait
-// ends.
+// indices and in the backward substitution every task's parents larger ones;
+// the wavefront whose next task comes first (last) in the sweep always finds
+// its dependencies done, so every wait ends.  A task at index q waits, for
+// each other wavefront, until it has passed the task nearest q that holds a
+// child (parent) of the task's fronts, from the per-residue distances of the
+// descriptor; with 1, 2 or 4 wavefronts a wavefront runs the tasks of one or
+// two residues, with 8 the task waits for every wavefront to pass the
+// farthest such task.
 __device__ inline void publish_progress(int* prog, int wave, int next) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this task's stores before the new index
     if (lane_id() == 0) __hip_atomic_store(prog + wave, next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 template <bool BWD, int NW>
-__device__ inline void wait_for_deps(int const* prog, int wave, GridTask const& tk) {
-    int const dep = BWD ? tk.dep_b : tk.dep_f;
-    if (BWD ? dep == 0x7fffffff : dep < 0) return;
+__device__ inline void wait_for_deps(int const* prog, int wave, GridTask const& tk, int q) {
+    static_assert(kDepWaves == 4 && (NW == 1 || NW == 2 || NW == 4 || NW == 8), "wavefronts per network");
+    unsigned const d01 = BWD ? tk.dep_b[0] : tk.dep_f[0], d23 = BWD ? tk.dep_b[1] : tk.dep_f[1];
+    if (NW == 1 || (d01 | d23) == 0u) return;
+    int const o[4] = {static_cast<int>(d01 & 0xffff), static_cast<int>(d01 >> 16), static_cast<int>(d23 & 0xffff),
+                      static_cast<int>(d23 >> 16)};
+    // the task index a wavefront must have passed, over residues r of mask
+    auto need = [&](unsigned mask, int& dep) {
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            if ((mask >> r & 1) && o[r]) {
+                int const d = BWD ? q + o[r] : q - o[r];
+                dep = !any ? d : (BWD ? min(dep, d) : max(dep, d));
+                any = true;
+            }
+        return any;
+    };
+#pragma unroll
     for (int w = 0; w < NW; ++w) {
         if (w == wave) continue;
+        int dep = 0;
+        if (!need(NW == 4 ? 1u << w : NW == 2 ? 5u << w : 15u, dep)) continue;
         for (;;) {
             int const at = __hip_atomic_load(prog + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (BWD ? at < dep : at > dep) break;
@@ -879,14 +907,14 @@ __device__ inline void run_sweep(FusedArgs const& a, FrontArgs const& fa, MisArg
     GridTask t0 = tasks[q], t1 = desc(q + S);
     TaskPre<T> p0 = load_pre<T, BWD>(a, t0);
     for (;;) {
-        wait_for_deps<BWD, NW>(prog, wave, t0);
+        wait_for_deps<BWD, NW>(prog, wave, t0, q);
         TaskPre<T> const p1 = load_pre<T, BWD>(a, t1);
         GridTask const t2 = desc(q + 2 * S);
         grid_task<T, BWD>(a, fa, ma, g, b0, r0, t0, p0, front, vec, xs, pool);
         q += S;
         publish_progress(prog, wave, q);
         if (!inside(q)) break;
-        wait_for_deps<BWD, NW>(prog, wave, t1);
+        wait_for_deps<BWD, NW>(prog, wave, t1, q);
         TaskPre<T> const p2 = load_pre<T, BWD>(a, t2);
         GridTask const t3 = desc(q + 2 * S);
         grid_task<T, BWD>(a, fa, ma, g, b0, r0, t1, p1, front, vec, xs, pool);
