@@ -420,12 +420,55 @@ __device__ inline T pivot_div(T x, T d) {
     else return x / d;
 }
 
+// A factor task's gathers that do not depend on other tasks: V and S of its
+// pairs' buses and the -F of its pivot equations (written by the mismatch
+// pass).  They are issued before the task waits for its children.
+template <typename T>
+struct TaskGath {
+    cplx_t<T> vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
+    T rv;
+};
+template <typename T>
+__device__ inline TaskGath<T> factor_gathers(FusedArgs const& a, MisArgs const& ma, GridTask const& tk,
+                                             TaskPre<T> const& pre, int b0, int r0) {
+    using C = cplx_t<T>;
+    constexpr int VS = sizeof(T) == 4 ? 2 : 1;  // FP32: V and S of a bus side by side
+    int const lane = lane_id();
+    C const* vsrc;
+    C const* ssrc;
+    if constexpr (sizeof(T) == 4) {
+        vsrc = reinterpret_cast<C const*>(ma.vsf + b0);
+        ssrc = vsrc + 1;
+    } else {
+        vsrc = ma.v + b0;
+        ssrc = ma.s + b0;
+    }
+    // unconditional: a lane whose value is not used reads lane 0's element
+    // instead, so its load falls into a line the wavefront fetches anyway
+    TaskGath<T> r;
+#pragma unroll
+    for (int k = 0; k < kPairChunks; ++k) {
+        bool const pok = tk.pair_beg + k * kWave + lane < tk.pair_end;
+        int const li = pre.pr[k].x & 0xffff, lj = static_cast<int>(static_cast<unsigned>(pre.pr[k].x) >> 16);
+        int const i0 = __builtin_amdgcn_readfirstlane(li);
+        bool const diag = li == lj;
+        r.vi[k] = vsrc[VS * (pok ? li : i0)];
+        r.vj[k] = vsrc[VS * (pok && !diag ? lj : i0)];
+        r.si[k] = ssrc[VS * (pok && diag ? li : i0)];
+    }
+    bool const rok = tk.rhs_beg + lane < tk.rhs_end;
+    T const* fe = static_cast<T const*>(a.fe) + r0;  // the network's equations
+    r.rv = fe[rok ? lidx(pre.rq) : __builtin_amdgcn_readfirstlane(lidx(pre.rq))];
+    return r;
+}
+
 // Assemble, factor and forward-eliminate up to kWave / L fronts of one class.
 // Stores the U rows (diagonal onwards), the Schur complement and the
 // eliminated right-hand side w of every front.
 template <typename T, int MM, int L>
 __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, GridTask const& tk,
-                                 TaskPre<T> const& pre, int g, int b0, int r0, T* front, T* vec, T* pool, int ld) {
+                                 TaskPre<T> const& pre, TaskGath<T> const& gat, int g, int b0, int r0, T* front, T* vec,
+                                 T* pool, int ld) {
     constexpr int G = kWave / L;
     int const lane = lane_id(), gi = lane / L, t = lane % L;
     int const rows = ld - 1, wbase = G * rows * ld, region = wbase + G * MM;
@@ -459,23 +502,16 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
     bool const vok = tk.vx_beg + lane < tk.vx_end;
     // a lane whose value is not used reads lane 0's element instead, so its
     // load falls into a line the wavefront fetches anyway
-    C vi[kPairChunks], vj[kPairChunks], si[kPairChunks];
-#pragma unroll
-    for (int k = 0; k < kPairChunks; ++k) {
-        int const li = pr[k].x & 0xffff, lj = static_cast<int>(static_cast<unsigned>(pr[k].x) >> 16);
-        int const i0 = __builtin_amdgcn_readfirstlane(li);
-        bool const diag = li == lj;
-        vi[k] = vsrc[VS * (pok[k] ? li : i0)];
-        vj[k] = vsrc[VS * (pok[k] && !diag ? lj : i0)];
-        si[k] = ssrc[VS * (pok[k] && diag ? li : i0)];
-    }
+    C const* vi = gat.vi;
+    C const* vj = gat.vj;
+    C const* si = gat.si;
     T exv[kExtChunks];
 #pragma unroll
     for (int k = 0; k < kExtChunks; ++k)
         exv[k] = front[eok[k] ? lidx(ex[k]) : __builtin_amdgcn_readfirstlane(lidx(ex[k]))];
     T const vxv = vec[vok ? lidx(vv) : __builtin_amdgcn_readfirstlane(lidx(vv))];
     T const* fe = static_cast<T const*>(a.fe) + r0;  // the network's equations
-    T const rv = fe[rok ? lidx(pre.rq) : __builtin_amdgcn_readfirstlane(lidx(pre.rq))];
+    T const rv = gat.rv;
     T* F = pool + gi * rows * ld;
     T* w = pool + wbase + gi * MM;
     wave_sync();  // the previous task of this wavefront has finished with the region
@@ -715,27 +751,27 @@ __device__ __attribute__((always_inline)) void grid_backward_task(FusedArgs cons
 
 template <typename T, bool BWD>
 __device__ inline void grid_task(FusedArgs const& a, FrontArgs const& fa, MisArgs const& ma, int g, int b0, int r0, GridTask const& tk,
-                                 TaskPre<T> const& pre, T* front, T* vec, T* xs, T* pool) {
+                                 TaskPre<T> const& pre, TaskGath<T> const& gat, T* front, T* vec, T* xs, T* pool) {
     switch (tk.cls_cnt >> 8) {
         case 0:
             if constexpr (BWD) grid_backward_task<T, 4, 4>(a, fa, tk, pre, front, vec, xs, pool, 5);
-            else grid_factor_task<T, 4, 4>(a, fa, ma, tk, pre, g, b0, r0, front, vec, pool, 5);
+            else grid_factor_task<T, 4, 4>(a, fa, ma, tk, pre, gat, g, b0, r0, front, vec, pool, 5);
             break;
         case 1:
             if constexpr (BWD) grid_backward_task<T, 8, 8>(a, fa, tk, pre, front, vec, xs, pool, 9);
-            else grid_factor_task<T, 8, 8>(a, fa, ma, tk, pre, g, b0, r0, front, vec, pool, 9);
+            else grid_factor_task<T, 8, 8>(a, fa, ma, tk, pre, gat, g, b0, r0, front, vec, pool, 9);
             break;
         case 2:
             if constexpr (BWD) grid_backward_task<T, 16, 16>(a, fa, tk, pre, front, vec, xs, pool, 17);
-            else grid_factor_task<T, 16, 16>(a, fa, ma, tk, pre, g, b0, r0, front, vec, pool, 17);
+            else grid_factor_task<T, 16, 16>(a, fa, ma, tk, pre, gat, g, b0, r0, front, vec, pool, 17);
             break;
         case 3:
             if constexpr (BWD) grid_backward_task<T, 32, 64>(a, fa, tk, pre, front, vec, xs, pool, 33);
-            else grid_factor_task<T, 32, 64>(a, fa, ma, tk, pre, g, b0, r0, front, vec, pool, 33);
+            else grid_factor_task<T, 32, 64>(a, fa, ma, tk, pre, gat, g, b0, r0, front, vec, pool, 33);
             break;
         default:
             if constexpr (BWD) grid_backward_task<T, 64, 64>(a, fa, tk, pre, front, vec, xs, pool, a.ld4);
-            else grid_factor_task<T, 64, 64>(a, fa, ma, tk, pre, g, b0, r0, front, vec, pool, a.ld4);
+            else grid_factor_task<T, 64, 64>(a, fa, ma, tk, pre, gat, g, b0, r0, front, vec, pool, a.ld4);
     }
 }
 
@@ -906,18 +942,24 @@ __device__ inline void run_sweep(FusedArgs const& a, FrontArgs const& fa, MisArg
     if (!inside(q)) return;
     GridTask t0 = tasks[q], t1 = desc(q + S);
     TaskPre<T> p0 = load_pre<T, BWD>(a, t0);
+    auto gathers = [&](GridTask const& tk, TaskPre<T> const& pre) {
+        if constexpr (BWD) return TaskGath<T>{};
+        else return factor_gathers<T>(a, ma, tk, pre, b0, r0);
+    };
     for (;;) {
+        TaskGath<T> const g0 = gathers(t0, p0);
         wait_for_deps<BWD, NW>(prog, wave, t0, q);
         TaskPre<T> const p1 = load_pre<T, BWD>(a, t1);
         GridTask const t2 = desc(q + 2 * S);
-        grid_task<T, BWD>(a, fa, ma, g, b0, r0, t0, p0, front, vec, xs, pool);
+        grid_task<T, BWD>(a, fa, ma, g, b0, r0, t0, p0, g0, front, vec, xs, pool);
         q += S;
         publish_progress(prog, wave, q);
         if (!inside(q)) break;
+        TaskGath<T> const g1 = gathers(t1, p1);
         wait_for_deps<BWD, NW>(prog, wave, t1, q);
         TaskPre<T> const p2 = load_pre<T, BWD>(a, t2);
         GridTask const t3 = desc(q + 2 * S);
-        grid_task<T, BWD>(a, fa, ma, g, b0, r0, t1, p1, front, vec, xs, pool);
+        grid_task<T, BWD>(a, fa, ma, g, b0, r0, t1, p1, g1, front, vec, xs, pool);
         q += S;
         publish_progress(prog, wave, q);
         if (!inside(q)) break;
