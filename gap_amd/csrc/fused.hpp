@@ -946,19 +946,21 @@ __device__ inline void run_sweep(FusedArgs const& a, FrontArgs const& fa, MisArg
         if constexpr (BWD) return TaskGath<T>{};
         else return factor_gathers<T>(a, ma, tk, pre, b0, r0);
     };
+    // (a task's own independent gathers and the next task's lists are in
+    // flight while it waits for its dependencies)
     for (;;) {
         TaskGath<T> const g0 = gathers(t0, p0);
-        wait_for_deps<BWD, NW>(prog, wave, t0, q);
         TaskPre<T> const p1 = load_pre<T, BWD>(a, t1);
         GridTask const t2 = desc(q + 2 * S);
+        wait_for_deps<BWD, NW>(prog, wave, t0, q);
         grid_task<T, BWD>(a, fa, ma, g, b0, r0, t0, p0, g0, front, vec, xs, pool);
         q += S;
         publish_progress(prog, wave, q);
         if (!inside(q)) break;
         TaskGath<T> const g1 = gathers(t1, p1);
-        wait_for_deps<BWD, NW>(prog, wave, t1, q);
         TaskPre<T> const p2 = load_pre<T, BWD>(a, t2);
         GridTask const t3 = desc(q + 2 * S);
+        wait_for_deps<BWD, NW>(prog, wave, t1, q);
         grid_task<T, BWD>(a, fa, ma, g, b0, r0, t1, p1, g1, front, vec, xs, pool);
         q += S;
         publish_progress(prog, wave, q);
