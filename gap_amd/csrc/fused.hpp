@@ -166,10 +166,10 @@ static_assert(kListPad >= kPairChunks * kWave && kListPad >= kExtChunks * kWave 
 // equation's -F is written in the factor type, by equation index (the factor
 // tasks gather it); the FP64 F only for the network's first equation in the
 // reference's numbering (rf in this schedule's numbering), stored at r0: the
-// one row the convergence test reads back.  The B rows'
-// loads are issued together, four entries per row and step; the sums keep the
-// row order.  B = 2 pays with FP64 factors; with FP32 factors its registers
-// would spill (the kernel is held to 128).
+// one row the convergence test reads back.  The B rows' loads are issued
+// together, four entries per row and step; the sums keep the row order.
+// B = 2 pays with FP64 factors; with FP32 factors its registers would spill
+// (that kernel is held to 96).
 template <typename T, int B>
 __device__ inline double grid_mismatch_buses(MisArgs const& a, int const (&i)[B], bool use_rest, int r0, int rf) {
 #pragma clang fp contract(off)
