@@ -644,7 +644,12 @@ __device__ __attribute__((always_inline)) void grid_factor_task(FusedArgs const&
         T const rp = pivot_recip(piv);
         if (k < p && t > k && t < m) {
             T const ukj = F[k * ld + t];
-            for (int i = k + 1; i < m; ++i) F[i * ld + t] -= (F[i * ld + k] * rp) * ukj;
+            // column k is read, column t > k written: they never overlap, so
+            // the reads of later rows need not wait for earlier rows' writes
+            T const* __restrict__ ck = F + k;
+            T* __restrict__ ct = F + t;
+#pragma unroll 4
+            for (int i = k + 1; i < m; ++i) ct[i * ld] -= (ck[i * ld] * rp) * ukj;
             w[t] -= (F[t * ld + k] * rp) * w[k];
         }
         wave_sync();
