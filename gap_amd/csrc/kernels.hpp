@@ -466,7 +466,12 @@ __global__ void __launch_bounds__(NT) k_front_factor_lds(FrontArgs a, int begin,
         T const rp = T(1) / piv;
         if (k < p && t > k && t < m) {
             T const ukj = F[k * LD + t];
-            for (int i = k + 1; i < m; ++i) F[i * LD + t] -= (F[i * LD + k] * rp) * ukj;
+            // column k is read, column t > k written: they never overlap, so
+            // the reads of later rows need not wait for earlier rows' writes
+            T const* __restrict__ ck = F + k;
+            T* __restrict__ ct = F + t;
+#pragma unroll 4
+            for (int i = k + 1; i < m; ++i) ct[i * LD] -= (ck[i * LD] * rp) * ukj;
         }
         __syncthreads();
         if (k < p && t == k)
