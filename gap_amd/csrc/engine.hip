@@ -291,7 +291,7 @@ struct GridItems {
     std::vector<int> const& bus_off;  // per instance, n_inst + 1 entries
     std::vector<int> const& var_off;  // per instance first Newton equation / variable, n_inst + 1 entries
     long long y_nnz;                  // Y-bus entries of the plan
-    // the grid schedule's own bus and equation order (grid_bus_order)
+    // the grid schedule's own bus and equation order (built in build_plan)
     std::vector<int> const& bus_new;  // per bus: its local index in that order
     std::vector<int> const& eq_new;   // per Newton equation: its local index in that order
     std::vector<int> const& var_bus;  // per Newton variable: its bus, as a global index in that order
