@@ -289,6 +289,24 @@ def test_grid_bus_order_is_internal(device):
     assert max(np.abs(a - b).max() for a, b in zip(rg["v"], rl["v"])) <= 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_grid_schedule_tiny_networks(device, precision):
+    """Networks of two and three buses (one front, or a few) batched with
+    larger ones: the grid schedule's own bus order and per-wavefront task
+    dependencies hold at the smallest sizes, equal to the level schedule."""
+    nets = [gridgen.small_mesh(2, seed=11, n_extra=0), gridgen.small_mesh(3, seed=12, n_extra=0),
+            gridgen.small_mesh(3, seed=13, n_extra=1)] + _batch(5, 901)
+    g = capi.Plan(nets, precision=precision, schedule="grid")
+    lv = capi.Plan(nets, precision=precision, schedule="levels")
+    assert g.schedule == "grid" and lv.schedule == "levels"
+    rg = g.solve(1e-8, 100, 100e6)
+    rl = lv.solve(1e-8, 100, 100e6)
+    assert rg["converged"].all() and rl["converged"].all()
+    assert np.array_equal(rg["iterations"], rl["iterations"])
+    for k in range(len(nets)):
+        assert np.abs(rg["v"][k] - rl["v"][k]).max() <= (1e-12 if precision == "fp64" else 1e-10)
+
+
 def test_grid_tuning_does_not_change_results(device):
     from gap_amd import tuning
     nets = _batch(12, 321) + [gridgen.small_mesh(4000, seed=5, n_extra=40, load_factor=0.5)]
