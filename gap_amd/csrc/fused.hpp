@@ -827,16 +827,34 @@ __global__ void __launch_bounds__(256) k_ybus_grid(YbusGridArgs a) {
 #pragma clang fp contract(off)
     int const k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.nnz) return;
-    double re = 0.0, im = 0.0;
-    for (int s = a.seg[k]; s < a.seg[k + 1]; ++s) {
-        int const c = a.code[s];
-        int const kind = c & 3, idx = c >> 2;
-        if (kind == 0) {
-            double2 const sh = a.app[idx];
-            re += sh.x;
-            im += sh.y;
+    int const s0 = a.seg[k], s1 = a.seg[k + 1];
+    int const inst = a.slot_inst[k];
+    int const q = a.slot_pair ? a.slot_pair[k] : -1;
+    // The first kHead contributions' codes and records are all loaded before
+    // any is summed, so a bus row of degree d costs three dependent round
+    // trips instead of 1 + 2d; the sum itself runs in code order.
+    constexpr int kHead = 4;
+    int c[kHead];
+#pragma unroll
+    for (int u = 0; u < kHead; ++u) c[u] = s0 + u < s1 ? a.code[s0 + u] : 0;
+    double4 rec[kHead];
+#pragma unroll
+    for (int u = 0; u < kHead; ++u) {
+        if (s0 + u >= s1) continue;
+        if ((c[u] & 3) == 0) {
+            double2 const sh = a.app[c[u] >> 2];
+            rec[u] = make_double4(sh.x, sh.y, 0.0, 0.0);
         } else {
-            double4 const p = a.brp[idx];
+            rec[u] = a.brp[c[u] >> 2];
+        }
+    }
+    double re = 0.0, im = 0.0;
+    auto add = [&](int code, double4 const& p) {
+        int const kind = code & 3;
+        if (kind == 0) {
+            re += p.x;
+            im += p.y;
+        } else {
             double const r = p.x, x = p.y;
             double const z2 = r * r + x * x;
             double yr = 0.0, yi = 0.0;
@@ -854,12 +872,23 @@ __global__ void __launch_bounds__(256) k_ybus_grid(YbusGridArgs a) {
                 im += ti;
             }
         }
+    };
+#pragma unroll
+    for (int u = 0; u < kHead; ++u)
+        if (s0 + u < s1) add(c[u], rec[u]);
+    for (int s = s0 + kHead; s < s1; ++s) {
+        int const cs = a.code[s];
+        if ((cs & 3) == 0) {
+            double2 const sh = a.app[cs >> 2];
+            add(cs, make_double4(sh.x, sh.y, 0.0, 0.0));
+        } else {
+            add(cs, a.brp[cs >> 2]);
+        }
     }
-    double const zb = a.zb[a.slot_inst[k]];
+    double const zb = a.zb[inst];
     double2 const y = make_double2(re * zb, im * zb);
     a.y[k] = y;
     if (a.slot_pair) {
-        int const q = a.slot_pair[k];
         if (q >= 0) {
             cplx_t<T>* yp = static_cast<cplx_t<T>*>(a.ypair);
             yp[q].x = static_cast<T>(y.x);
