@@ -50,17 +50,39 @@ struct YbusArgs {
 };
 
 // One Y slot: its contributions in the builder's order (segment k of code).
+// The first kHead contributions' codes and parameters are all loaded before
+// any is summed, so a bus row of degree d costs three dependent round trips
+// instead of 1 + 2d; the sum itself runs in code order.
 __device__ inline void ybus_slot(YbusArgs const& a, int k) {
 #pragma clang fp contract(off)
-    double re = 0.0, im = 0.0;
-    for (int s = a.seg[k]; s < a.seg[k + 1]; ++s) {
-        int c = a.code[s];
-        int kind = c & 3, idx = c >> 2;
-        if (kind == 0) {
-            re += a.app_g[idx];
-            im += a.app_b[idx];
+    int const s0 = a.seg[k], s1 = a.seg[k + 1];
+    int const inst = a.slot_inst[k];
+    constexpr int kHead = 4;
+    int c[kHead];
+#pragma unroll
+    for (int u = 0; u < kHead; ++u) c[u] = s0 + u < s1 ? a.code[s0 + u] : 0;
+    // Per contribution: a shunt's (g, b), or a branch's (r, x, g, b).
+    double4 par[kHead];
+#pragma unroll
+    for (int u = 0; u < kHead; ++u) {
+        par[u] = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (s0 + u >= s1) continue;
+        int const idx = c[u] >> 2;
+        if ((c[u] & 3) == 0) {
+            par[u].x = a.app_g[idx];
+            par[u].y = a.app_b[idx];
         } else {
-            double r = a.br_r[idx], x = a.br_x[idx];
+            par[u] = make_double4(a.br_r[idx], a.br_x[idx], a.br_g[idx], a.br_b[idx]);
+        }
+    }
+    double re = 0.0, im = 0.0;
+    auto add = [&](int code, double4 const& p) {
+        int const kind = code & 3;
+        if (kind == 0) {
+            re += p.x;
+            im += p.y;
+        } else {
+            double r = p.x, x = p.y;
             double z2 = r * r + x * x;
             double yr = 0.0, yi = 0.0;
             if (z2 > 1e-12) {
@@ -71,15 +93,25 @@ __device__ inline void ybus_slot(YbusArgs const& a, int k) {
                 re = -yr;
                 im = -yi;
             } else {
-                double bar = yr + a.br_g[idx], bai = yi + 0.0;
-                double tr = bar + 0.0, ti = bai + a.br_b[idx] / 2.0;
+                double bar = yr + p.z, bai = yi + 0.0;
+                double tr = bar + 0.0, ti = bai + p.w / 2.0;
                 re += tr;
                 im += ti;
             }
         }
+    };
+#pragma unroll
+    for (int u = 0; u < kHead; ++u)
+        if (s0 + u < s1) add(c[u], par[u]);
+    for (int s = s0 + kHead; s < s1; ++s) {
+        int const cs = a.code[s], idx = cs >> 2;
+        if ((cs & 3) == 0)
+            add(cs, make_double4(a.app_g[idx], a.app_b[idx], 0.0, 0.0));
+        else
+            add(cs, make_double4(a.br_r[idx], a.br_x[idx], a.br_g[idx], a.br_b[idx]));
     }
     if (a.y_si) a.y_si[k] = make_double2(re, im);
-    double zb = a.zb[a.slot_inst[k]];
+    double zb = a.zb[inst];
     a.y_pu[k] = make_double2(re * zb, im * zb);
 }
 
